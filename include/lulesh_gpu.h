@@ -1,0 +1,174 @@
+/* lulesh_gpu.h -- C ABI of the MI355X (gfx950) LULESH Lagrange leapfrog step.
+ *
+ * The reference has no plugin or FFI interface: its hot path is the in-file
+ * call structure LagrangeLeapFrog(Domain&) -> LagrangeNodal(Domain&),
+ * LagrangeElements(Domain&, Index_t), CalcTimeConstraintsForElems(Domain&)
+ * (lulesh.cc:2600-2645), all of it operating on the Domain data contract
+ * (lulesh.h:148-595).  This header is the seam that replaces those calls: a
+ * host Domain hands its mesh, connectivity, region sets and state to an opaque
+ * device context once, then calls the three phases per cycle.
+ *
+ * Conventions
+ *   - Plain pointers and sizes only.  Host arrays are read during the call
+ *     and not retained.
+ *   - Real fields are FP64 on the host side of the ABI in every precision
+ *     mode; the mixed mode stores some of them in FP32 on the device and
+ *     converts on upload/download.
+ *   - lgpu_lagrange_nodal and lgpu_lagrange_elements only enqueue work on the
+ *     context's stream.  lgpu_time_constraints is the per-cycle sync point: it
+ *     returns the new dt constraints together with the status of every check
+ *     the reference makes with exit(): LGPU_VOLUME_ERROR (-1) or
+ *     LGPU_QSTOP_ERROR (-2), as VolumeError/QStopError in lulesh.h:42, and in
+ *     the order the reference would have hit them.
+ *   - One context per host thread; calls on one context are not re-entrant.
+ *   - Return codes: 0 ok, -1/-2 as above, <= -100 usage or device errors.
+ */
+#ifndef LULESH_GPU_H
+#define LULESH_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lgpu_ctx lgpu_ctx;
+
+enum lgpu_status {
+   LGPU_OK = 0,
+   LGPU_VOLUME_ERROR = -1, /* VolumeError, lulesh.h:42 */
+   LGPU_QSTOP_ERROR = -2,  /* QStopError,  lulesh.h:42 */
+   LGPU_EINVAL = -100,     /* bad argument or mesh description */
+   LGPU_EDEVICE = -101,    /* HIP runtime error */
+   LGPU_ENOMEM = -102      /* device allocation failed */
+};
+
+enum lgpu_precision {
+   LGPU_FP64 = 0,  /* every quantity in FP64, arithmetic as the reference */
+   LGPU_MIXED = 1  /* FP32 force integration, Q gradients, EOS pressure and
+                      sound speed; FP64 node integration, volumes, energy
+                      update and dt reductions */
+};
+
+/* Real fields (Domain accessors of lulesh.h:266-367).  Node fields first. */
+enum lgpu_field {
+   LGPU_X = 0, LGPU_Y, LGPU_Z, LGPU_XD, LGPU_YD, LGPU_ZD, LGPU_XDD, LGPU_YDD, LGPU_ZDD,
+   LGPU_FX, LGPU_FY, LGPU_FZ, LGPU_NODALMASS,
+   LGPU_E, LGPU_P, LGPU_Q, LGPU_QL, LGPU_QQ, LGPU_V, LGPU_VOLO, LGPU_VNEW, LGPU_DELV,
+   LGPU_VDOV, LGPU_AREALG, LGPU_SS, LGPU_ELEMMASS,
+   LGPU_DELX_XI, LGPU_DELX_ETA, LGPU_DELX_ZETA, LGPU_DELV_XI, LGPU_DELV_ETA, LGPU_DELV_ZETA,
+   LGPU_NUM_FIELDS
+};
+
+/* Physics constants (Domain constructor, lulesh-init.cc:20-38). */
+typedef struct lgpu_constants {
+   double e_cut, p_cut, q_cut, v_cut, u_cut;
+   double hgcoef, ss4o3, qstop, monoq_max_slope, monoq_limiter_mult;
+   double qlc_monoq, qqc_monoq, qqc, eosvmax, eosvmin, pmin, emin, dvovmax, refdens;
+} lgpu_constants;
+
+/* Mesh, connectivity and region description: the integer part of Domain
+ * (lulesh.h:478-500, 587-588) plus the EOS cost knob. */
+typedef struct lgpu_mesh_desc {
+   int32_t numElem, numNode;
+   int32_t sizeX, sizeY, sizeZ;               /* elements per edge           */
+   const int32_t *nodelist;                   /* 8*numElem                   */
+   const int32_t *lxim, *lxip, *letam, *letap, *lzetam, *lzetap; /* numElem  */
+   const int32_t *elemBC;                     /* numElem                     */
+   const int32_t *nodeElemStart;              /* numNode+1                   */
+   const int32_t *nodeElemCornerList;         /* nodeElemStart[numNode]      */
+   const int32_t *symmX, *symmY, *symmZ;      /* node sets, may be NULL      */
+   int32_t numSymmX, numSymmY, numSymmZ;
+   int32_t numReg;
+   const int32_t *regElemSize;                /* numReg                      */
+   const int32_t *const *regElemlist;         /* numReg lists                */
+   int32_t cost;                              /* EOS repeat rule, lulesh.cc:2390-2400 */
+   int32_t precision;                         /* enum lgpu_precision         */
+   lgpu_constants k;
+} lgpu_mesh_desc;
+
+/* Fills *k with the reference's constants (lulesh-init.cc:20-38). */
+void lgpu_default_constants(lgpu_constants *k);
+
+/* Builds the device context: validates the description, allocates one device
+ * arena for every array the step needs (the reference mallocs its scratch
+ * per cycle, lulesh.cc:514-518, 737-741, 1001-1006, 2222-2235), uploads the
+ * connectivity and picks the kernel path.  Replaces the device-relevant part
+ * of Domain::Domain (lulesh-init.cc:16-194).  State fields start at zero
+ * except v = 1; upload the rest with lgpu_upload. */
+int lgpu_create(const lgpu_mesh_desc *desc, lgpu_ctx **out);
+
+/* Frees the context (Domain::~Domain, lulesh-init.cc:198-213). */
+void lgpu_destroy(lgpu_ctx *ctx);
+
+/* Synchronous host -> device copy of one real field (count = numNode or
+ * numElem values). */
+int lgpu_upload(lgpu_ctx *ctx, int field, const double *host);
+
+/* Synchronous device -> host copy of one real field.  Fields the step does
+ * not keep (fx.., xdd.., ql, qq) hold their last value only while
+ * lgpu_set_option(LGPU_OPT_KEEP_DIAGNOSTICS) is on. */
+int lgpu_download(lgpu_ctx *ctx, int field, double *host);
+
+/* LagrangeNodal (lulesh.cc:1223-1269): nodal forces (stress integration and
+ * Flanagan-Belytschko hourglass control, summed per node in
+ * nodeElemCornerList order), acceleration, symmetry-plane BCs, velocity and
+ * position, with dt = domain.deltatime().  Enqueues only. */
+int lgpu_lagrange_nodal(lgpu_ctx *ctx, double dt);
+
+/* LagrangeElements (lulesh.cc:2431-2443): kinematics, monotonic Q, the
+ * region-by-region EOS with its repeat counts, and the volume update.
+ * Enqueues only. */
+int lgpu_lagrange_elements(lgpu_ctx *ctx, double dt);
+
+/* CalcTimeConstraintsForElems (lulesh.cc:2576-2596): Courant and hydro dt
+ * minima over all regions.  Waits for the cycle to finish and returns its
+ * status (0, -1 or -2); on 0 writes *dtcourant and *dthydro. */
+int lgpu_time_constraints(lgpu_ctx *ctx, double *dtcourant, double *dthydro);
+
+/* Time-step controls of Domain (lulesh.h:557-567) used by the device-side
+ * loop below. */
+typedef struct lgpu_time_state {
+   double time, deltatime, dtcourant, dthydro, stoptime, dtfixed, dtmax;
+   double deltatimemultlb, deltatimemultub;
+   int32_t cycle;
+} lgpu_time_state;
+
+/* The reference's main loop (lulesh.cc:2745-2748) with TimeIncrement
+ * (lulesh.cc:166-222) evaluated on the device: runs while time < stoptime and
+ * cycle < max_cycle without a host round trip per cycle, then writes the
+ * time state back to *ts.  Returns the status as lgpu_time_constraints. */
+int lgpu_run(lgpu_ctx *ctx, lgpu_time_state *ts, int32_t max_cycle);
+
+/* Options. */
+enum lgpu_option {
+   LGPU_OPT_KEEP_DIAGNOSTICS = 1, /* also store fx,fy,fz,xdd,ydd,zdd,ql,qq  */
+   LGPU_OPT_KERNEL_TIMING = 2,    /* time every kernel with HIP events       */
+   LGPU_OPT_BLOCK_SIZE = 3        /* threads per block of the element/node
+                                     kernels: 64, 128, 256, 512 or 1024      */
+};
+int lgpu_set_option(lgpu_ctx *ctx, int option, int64_t value);
+
+/* Kernel timing (LGPU_OPT_KERNEL_TIMING).  Kernels are numbered
+ * 0..lgpu_num_kernels()-1. */
+int lgpu_num_kernels(void);
+const char *lgpu_kernel_name(int kernel);
+int lgpu_kernel_stats(lgpu_ctx *ctx, int kernel, int64_t *launches, double *total_ms);
+int lgpu_reset_kernel_stats(lgpu_ctx *ctx);
+
+/* Context facts for reports: path (0 generic CSR gather, 1 structured
+ * brick), device arena bytes, precision. */
+typedef struct lgpu_info {
+   int32_t path, precision, numElem, numNode, numReg, block;
+   int64_t arena_bytes;
+} lgpu_info;
+int lgpu_get_info(lgpu_ctx *ctx, lgpu_info *info);
+
+/* Last HIP error text of this context (empty if none). */
+const char *lgpu_last_error(lgpu_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,65 @@
+/* lulesh_host.h -- C API of the host-side Domain (lulesh_amd/csrc/host).
+ *
+ * A flat C view of the C++ host Domain and its driver loop, for callers that
+ * are not C++ (the Python package, benchmarks).  Each function names the
+ * reference piece it stands for.  Real fields use the lgpu_field numbering of
+ * lulesh_gpu.h.
+ */
+#ifndef LULESH_HOST_H
+#define LULESH_HOST_H
+
+#include <stdint.h>
+
+#include "lulesh_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct lh_domain lh_domain;
+
+/* new Domain(1, 0, 0, 0, s, 1, nr, balance, cost) (lulesh.cc:2715-2716),
+ * optionally with the deposited energy scaled by emult. */
+lh_domain *lh_create(int32_t s, int32_t nr, int32_t balance, int32_t cost, double emult);
+void lh_destroy(lh_domain *d);
+
+/* Creates the device context for this domain and uploads its state. */
+int lh_attach_device(lh_domain *d, int32_t precision);
+lgpu_ctx *lh_device(lh_domain *d);
+
+/* One iteration of the main loop: TimeIncrement + LagrangeLeapFrog
+ * (lulesh.cc:2747-2748).  Returns 0 or the error code. */
+int lh_step(lh_domain *d);
+
+/* The main loop (lulesh.cc:2745-2757) until time >= stoptime or cycle >= its.
+ * device_loop = 1 runs TimeIncrement on the device (lgpu_run).  Returns 0 or
+ * the error code. */
+int lh_run(lh_domain *d, int32_t its, int32_t device_loop);
+
+/* Copies every state field from the device into the host Domain. */
+int lh_sync_from_device(lh_domain *d);
+
+int32_t lh_num_elem(lh_domain *d);
+int32_t lh_num_node(lh_domain *d);
+int32_t lh_num_reg(lh_domain *d);
+double *lh_real(lh_domain *d, int32_t field);
+
+/* Integer arrays: 0 nodelist, 1..6 lxim..lzetap, 7 elemBC, 8 nodeElemStart,
+ * 9 nodeElemCornerList, 10..12 symmX..Z, 13 regNumList, 14 regElemSize,
+ * 15 regElemlist[region]. */
+int32_t *lh_int(lh_domain *d, int32_t which, int32_t region, int64_t *len);
+
+/* Scalars: 0 time, 1 deltatime, 2 dtcourant, 3 dthydro, 4 cycle,
+ * 5 stoptime, 6 dtfixed, 7 dtmax, 8 deltatimemultlb, 9 deltatimemultub. */
+double lh_get(lh_domain *d, int32_t which);
+void lh_set(lh_domain *d, int32_t which, double value);
+
+/* VerifyAndWriteFinalOutput's plane-0 symmetry figures
+ * (lulesh-util.cc:197-212) on the host copy of e. */
+void lh_symmetry(lh_domain *d, double out[3]);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

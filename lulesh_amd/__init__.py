@@ -1,0 +1,11 @@
+"""MI355X (gfx950) implementation of the LULESH 2.x Lagrange leapfrog step.
+
+The device path is hand-written HIP behind a C ABI (include/lulesh_gpu.h);
+the host keeps the reference's Domain class and per-cycle call structure
+(include/lulesh_host.h).  This package only loads those libraries.
+"""
+from .native import (FIELDS, FP64, MIXED, DeviceContext, HostDomain, TimeState,  # noqa: F401
+                     check, default_constants, gpu_lib, host_lib)
+
+__all__ = ["FIELDS", "FP64", "MIXED", "DeviceContext", "HostDomain", "TimeState", "check",
+           "default_constants", "gpu_lib", "host_lib"]

@@ -1,0 +1,574 @@
+// domain.cc -- set-up of the host Domain, the per-cycle call structure over
+// the device path, and the flat C API (include/lulesh_host.h).
+#include "host/domain.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include <algorithm>
+
+#include "host/glibc_rand.h"
+#include "lulesh_host.h"
+
+namespace {
+
+// elemBC bits, lulesh.h:59-87.
+enum : Int_t {
+   XI_M_SYMM = 0x00001, XI_P_FREE = 0x00010, ETA_M_SYMM = 0x00040, ETA_P_FREE = 0x00400,
+   ZETA_M_SYMM = 0x01000, ZETA_P_FREE = 0x10000
+};
+
+} // namespace
+
+Domain::Domain(Int_t numRanks, Index_t colLoc, Index_t rowLoc, Index_t planeLoc, Index_t nx,
+               Int_t tp, Int_t nr, Int_t balance, Int_t cost)
+{
+   if (numRanks != 1 || tp != 1 || colLoc != 0 || rowLoc != 0 || planeLoc != 0 || nx < 1 || nr < 1) {
+      fprintf(stderr, "Domain: only a single domain (numRanks = tp = 1) is supported\n");
+      exit(-1);
+   }
+   lgpu_default_constants(&m_k);
+   const Index_t edgeElems = nx, edgeNodes = nx + 1;
+   m_cost = cost;
+   m_sizeX = m_sizeY = m_sizeZ = edgeElems;
+   m_numElem = edgeElems * edgeElems * edgeElems;
+   m_numNode = edgeNodes * edgeNodes * edgeNodes;
+   const size_t E = (size_t)m_numElem, N = (size_t)m_numNode;
+
+   for (auto *a : {&m_x, &m_y, &m_z, &m_xd, &m_yd, &m_zd, &m_xdd, &m_ydd, &m_zdd, &m_fx, &m_fy,
+                   &m_fz, &m_nodalMass})
+      a->assign(N, 0.0);
+   for (auto *a : {&m_e, &m_p, &m_q, &m_ql, &m_qq, &m_volo, &m_vnew, &m_delv, &m_vdov, &m_arealg,
+                   &m_ss, &m_elemMass, &m_delx_xi, &m_delx_eta, &m_delx_zeta, &m_delv_xi,
+                   &m_delv_eta, &m_delv_zeta})
+      a->assign(E, 0.0);
+   m_v.assign(E, 1.0); // v starts at 1 (lulesh-init.cc:97-100)
+   m_nodelist.assign(8 * E, 0);
+   for (auto *a : {&m_lxim, &m_lxip, &m_letam, &m_letap, &m_lzetam, &m_lzetap, &m_elemBC})
+      a->assign(E, 0);
+   m_regNumList.assign(E, 0);
+   m_symmX.assign((size_t)edgeNodes * edgeNodes, 0);
+   m_symmY.assign((size_t)edgeNodes * edgeNodes, 0);
+   m_symmZ.assign((size_t)edgeNodes * edgeNodes, 0);
+
+   BuildMesh(nx, edgeNodes, edgeElems);
+   SetupThreadSupportStructures();
+   CreateRegionIndexSets(nr, balance);
+   SetupSymmetryPlanes(edgeNodes);
+   SetupElementConnectivities(edgeElems);
+   SetupBoundaryConditions(edgeElems);
+
+   // Time controls, lulesh-init.cc:146-156.
+   m_dtfixed = -1.0e-6;
+   m_stoptime = 1.0e-2;
+   m_deltatimemultlb = 1.1;
+   m_deltatimemultub = 1.2;
+   m_dtcourant = 1.0e+20;
+   m_dthydro = 1.0e+20;
+   m_dtmax = 1.0e-2;
+   m_time = 0.;
+   m_cycle = 0;
+
+   // Reference volume and masses, lulesh-init.cc:158-178.
+   for (Index_t i = 0; i < m_numElem; ++i) {
+      const Index_t *nl = nodelist(i);
+      double xl[8], yl[8], zl[8];
+      for (int j = 0; j < 8; ++j) {
+         xl[j] = m_x[nl[j]];
+         yl[j] = m_y[nl[j]];
+         zl[j] = m_z[nl[j]];
+      }
+      // Hexahedron volume, lulesh.cc:1273-1356.
+      auto d = [](const double *c, int a, int b) { return c[a] - c[b]; };
+      auto trip = [](double ux, double uy, double uz, double vx, double vy, double vz, double wx,
+                     double wy, double wz) {
+         return ux * (vy * wz - vz * wy) + uy * (vz * wx - vx * wz) + uz * (vx * wy - vy * wx);
+      };
+      const double t1 = trip(d(xl, 3, 1) + d(xl, 7, 2), d(yl, 3, 1) + d(yl, 7, 2), d(zl, 3, 1) + d(zl, 7, 2),
+                             d(xl, 6, 3), d(yl, 6, 3), d(zl, 6, 3), d(xl, 2, 0), d(yl, 2, 0), d(zl, 2, 0));
+      const double t2 = trip(d(xl, 4, 3) + d(xl, 5, 7), d(yl, 4, 3) + d(yl, 5, 7), d(zl, 4, 3) + d(zl, 5, 7),
+                             d(xl, 6, 4), d(yl, 6, 4), d(zl, 6, 4), d(xl, 7, 0), d(yl, 7, 0), d(zl, 7, 0));
+      const double t3 = trip(d(xl, 1, 4) + d(xl, 2, 5), d(yl, 1, 4) + d(yl, 2, 5), d(zl, 1, 4) + d(zl, 2, 5),
+                             d(xl, 6, 1), d(yl, 6, 1), d(zl, 6, 1), d(xl, 5, 0), d(yl, 5, 0), d(zl, 5, 0));
+      double volume = t1 + t2 + t3;
+      volume *= 1.0 / 12.0;
+      m_volo[i] = volume;
+      m_elemMass[i] = volume;
+      for (int j = 0; j < 8; ++j) m_nodalMass[nl[j]] += volume / 8.0;
+   }
+   ScaleInitialEnergy(1.0);
+}
+
+Domain::~Domain()
+{
+   if (m_gpu) lgpu_destroy(m_gpu);
+}
+
+void Domain::ScaleInitialEnergy(Real_t m)
+{
+   // Energy deposit and first dt, lulesh-init.cc:180-192.
+   const Real_t ebase = 3.948746e+7;
+   const Real_t scale = m_sizeX / 45.0;
+   Real_t einit = ebase * scale * scale * scale;
+   if (m != 1.0) einit = einit * m;
+   m_e[0] = einit;
+   m_deltatime = (0.5 * cbrt(m_volo[0])) / sqrt(2.0 * einit);
+}
+
+// Node lattice and element-to-node map, lulesh-init.cc:217-267.
+void Domain::BuildMesh(Int_t nx, Int_t edgeNodes, Int_t edgeElems)
+{
+   // Coordinate of lattice line i: 1.125 * i / nx (0 for i = 0).
+   std::vector<Real_t> c((size_t)edgeNodes);
+   for (Index_t i = 0; i < edgeNodes; ++i) c[i] = 1.125 * (double)i / (double)nx;
+   Index_t n = 0;
+   for (Index_t k = 0; k < edgeNodes; ++k)
+      for (Index_t j = 0; j < edgeNodes; ++j)
+         for (Index_t i = 0; i < edgeNodes; ++i, ++n) {
+            m_x[n] = c[i];
+            m_y[n] = c[j];
+            m_z[n] = c[k];
+         }
+   const Index_t en = edgeNodes, en2 = edgeNodes * edgeNodes;
+   Index_t e = 0;
+   for (Index_t k = 0; k < edgeElems; ++k)
+      for (Index_t j = 0; j < edgeElems; ++j)
+         for (Index_t i = 0; i < edgeElems; ++i, ++e) {
+            const Index_t b = k * en2 + j * en + i;
+            Index_t *nl = nodelist(e);
+            nl[0] = b;
+            nl[1] = b + 1;
+            nl[2] = b + en + 1;
+            nl[3] = b + en;
+            nl[4] = b + en2;
+            nl[5] = b + en2 + 1;
+            nl[6] = b + en2 + en + 1;
+            nl[7] = b + en2 + en;
+         }
+}
+
+// Node-to-corner lists, lulesh-init.cc:271-337: corners of each node in
+// ascending (element, corner) order.  Always built, since the device gather
+// uses them.
+void Domain::SetupThreadSupportStructures()
+{
+   const size_t N = (size_t)m_numNode;
+   std::vector<Index_t> count(N, 0);
+   for (Index_t n : m_nodelist) ++count[(size_t)n];
+   m_nodeElemStart.assign(N + 1, 0);
+   for (size_t n = 0; n < N; ++n) m_nodeElemStart[n + 1] = m_nodeElemStart[n] + count[n];
+   m_nodeElemCornerList.assign((size_t)m_nodeElemStart[N], 0);
+   std::fill(count.begin(), count.end(), 0);
+   for (size_t c = 0; c < m_nodelist.size(); ++c) {
+      const Index_t n = m_nodelist[c];
+      m_nodeElemCornerList[(size_t)(m_nodeElemStart[n] + count[(size_t)n]++)] = (Index_t)c;
+   }
+}
+
+// Region index sets, lulesh-init.cc:400-510 (single rank: seed 0, no
+// rotation), drawn from the glibc sequence.
+void Domain::CreateRegionIndexSets(Int_t nr, Int_t balance)
+{
+   lhost::GlibcRand rng(0);
+   m_numReg = nr;
+   m_regElemSize.assign((size_t)nr, 0);
+   m_regElemlist.assign((size_t)nr, {});
+   Index_t next = 0;
+   if (nr == 1) {
+      std::fill(m_regNumList.begin(), m_regNumList.end(), 1);
+   } else {
+      // Cumulative weights (i+1)^balance.
+      std::vector<Int_t> binEnd((size_t)nr);
+      Int_t total = 0;
+      for (Int_t i = 0; i < nr; ++i) {
+         total += pow(i + 1, balance);
+         binEnd[(size_t)i] = total;
+      }
+      auto draw_region = [&]() {
+         const Int_t v = rng.next() % total;
+         Int_t i = 0;
+         while (v >= binEnd[(size_t)i]) ++i;
+         return (i % nr) + 1;
+      };
+      Int_t last = -1;
+      while (next < m_numElem) {
+         Int_t reg = draw_region();
+         while (reg == last) reg = draw_region();
+         // Run length by bin (lulesh-init.cc:460-480).
+         const Int_t bin = rng.next() % 1000;
+         Index_t run;
+         if (bin < 773) run = rng.next() % 15 + 1;
+         else if (bin < 937) run = rng.next() % 16 + 16;
+         else if (bin < 970) run = rng.next() % 32 + 32;
+         else if (bin < 974) run = rng.next() % 64 + 64;
+         else if (bin < 978) run = rng.next() % 128 + 128;
+         else if (bin < 981) run = rng.next() % 256 + 256;
+         else run = rng.next() % 1537 + 512;
+         const Index_t end = std::min(next + run, m_numElem);
+         for (; next < end; ++next) m_regNumList[(size_t)next] = reg;
+         last = reg;
+      }
+   }
+   for (Index_t i = 0; i < m_numElem; ++i) m_regElemSize[(size_t)(m_regNumList[(size_t)i] - 1)]++;
+   for (Int_t r = 0; r < nr; ++r) m_regElemlist[(size_t)r].reserve((size_t)m_regElemSize[(size_t)r]);
+   for (Index_t i = 0; i < m_numElem; ++i) m_regElemlist[(size_t)(m_regNumList[(size_t)i] - 1)].push_back(i);
+}
+
+// Symmetry-plane node sets, lulesh-init.cc:513-533.
+void Domain::SetupSymmetryPlanes(Int_t edgeNodes)
+{
+   Index_t n = 0;
+   for (Index_t a = 0; a < edgeNodes; ++a)
+      for (Index_t b = 0; b < edgeNodes; ++b, ++n) {
+         m_symmZ[(size_t)n] = a * edgeNodes + b;
+         m_symmY[(size_t)n] = a * edgeNodes * edgeNodes + b;
+         m_symmX[(size_t)n] = a * edgeNodes * edgeNodes + b * edgeNodes;
+      }
+}
+
+// Face neighbours, lulesh-init.cc:538-565: the neighbouring element across
+// each face, or the element itself on the mesh boundary.
+void Domain::SetupElementConnectivities(Int_t s)
+{
+   const Index_t s2 = s * s;
+   for (Index_t e = 0; e < m_numElem; ++e) {
+      // Like the reference, xi neighbours run along the flat element index
+      // (wrapping across rows); the boundary masks keep the wrapped ones
+      // from being read.
+      m_lxim[(size_t)e] = (e > 0) ? e - 1 : 0;
+      m_lxip[(size_t)e] = (e + 1 < m_numElem) ? e + 1 : e;
+      m_letam[(size_t)e] = (e >= s) ? e - s : e;
+      m_letap[(size_t)e] = (e + s < m_numElem) ? e + s : e;
+      m_lzetam[(size_t)e] = (e >= s2) ? e - s2 : e;
+      m_lzetap[(size_t)e] = (e + s2 < m_numElem) ? e + s2 : e;
+   }
+}
+
+// Boundary masks of a single domain, lulesh-init.cc:568-673: symmetry on the
+// three planes through the origin, free surface on the other three.
+void Domain::SetupBoundaryConditions(Int_t s)
+{
+   for (Index_t e = 0; e < m_numElem; ++e) {
+      const Index_t i = e % s, j = (e / s) % s, k = e / (s * s);
+      Int_t bc = 0;
+      if (i == 0) bc |= XI_M_SYMM;
+      if (i == s - 1) bc |= XI_P_FREE;
+      if (j == 0) bc |= ETA_M_SYMM;
+      if (j == s - 1) bc |= ETA_P_FREE;
+      if (k == 0) bc |= ZETA_M_SYMM;
+      if (k == s - 1) bc |= ZETA_P_FREE;
+      m_elemBC[(size_t)e] = bc;
+   }
+}
+
+Real_t *Domain::realField(int f)
+{
+   std::vector<Real_t> *t[LGPU_NUM_FIELDS] = {
+      &m_x, &m_y, &m_z, &m_xd, &m_yd, &m_zd, &m_xdd, &m_ydd, &m_zdd, &m_fx, &m_fy, &m_fz,
+      &m_nodalMass, &m_e, &m_p, &m_q, &m_ql, &m_qq, &m_v, &m_volo, &m_vnew, &m_delv, &m_vdov,
+      &m_arealg, &m_ss, &m_elemMass, &m_delx_xi, &m_delx_eta, &m_delx_zeta, &m_delv_xi,
+      &m_delv_eta, &m_delv_zeta};
+   return (f >= 0 && f < LGPU_NUM_FIELDS) ? t[f]->data() : nullptr;
+}
+
+namespace {
+// Fields that carry state from one cycle to the next (plus the set-up
+// constants volo, elemMass, nodalMass).
+const int kStateFields[] = {LGPU_X, LGPU_Y, LGPU_Z, LGPU_XD, LGPU_YD, LGPU_ZD, LGPU_NODALMASS,
+                            LGPU_E, LGPU_P, LGPU_Q, LGPU_V, LGPU_VOLO, LGPU_SS, LGPU_ELEMMASS};
+const int kDerivedFields[] = {LGPU_VNEW, LGPU_DELV, LGPU_VDOV, LGPU_AREALG};
+} // namespace
+
+int Domain::AttachDevice(int precision)
+{
+   if (m_gpu) {
+      lgpu_destroy(m_gpu);
+      m_gpu = nullptr;
+   }
+   std::vector<const Index_t *> lists((size_t)m_numReg);
+   for (Int_t r = 0; r < m_numReg; ++r) lists[(size_t)r] = m_regElemlist[(size_t)r].data();
+   lgpu_mesh_desc d{};
+   d.numElem = m_numElem;
+   d.numNode = m_numNode;
+   d.sizeX = m_sizeX;
+   d.sizeY = m_sizeY;
+   d.sizeZ = m_sizeZ;
+   d.nodelist = m_nodelist.data();
+   d.lxim = m_lxim.data();
+   d.lxip = m_lxip.data();
+   d.letam = m_letam.data();
+   d.letap = m_letap.data();
+   d.lzetam = m_lzetam.data();
+   d.lzetap = m_lzetap.data();
+   d.elemBC = m_elemBC.data();
+   d.nodeElemStart = m_nodeElemStart.data();
+   d.nodeElemCornerList = m_nodeElemCornerList.data();
+   d.symmX = m_symmX.data();
+   d.symmY = m_symmY.data();
+   d.symmZ = m_symmZ.data();
+   d.numSymmX = (int32_t)m_symmX.size();
+   d.numSymmY = (int32_t)m_symmY.size();
+   d.numSymmZ = (int32_t)m_symmZ.size();
+   d.numReg = m_numReg;
+   d.regElemSize = m_regElemSize.data();
+   d.regElemlist = lists.data();
+   d.cost = m_cost;
+   d.precision = precision;
+   d.k = m_k;
+   int st = lgpu_create(&d, &m_gpu);
+   if (st) return st;
+   for (int f : kStateFields) {
+      st = lgpu_upload(m_gpu, f, realField(f));
+      if (st) return st;
+   }
+   return LGPU_OK;
+}
+
+int Domain::SyncFromDevice()
+{
+   if (!m_gpu) return LGPU_EINVAL;
+   for (int f : kStateFields) {
+      const int st = lgpu_download(m_gpu, f, realField(f));
+      if (st) return st;
+   }
+   for (int f : kDerivedFields) {
+      const int st = lgpu_download(m_gpu, f, realField(f));
+      if (st) return st;
+   }
+   return LGPU_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Per-cycle call structure.
+
+void TimeIncrement(Domain &domain)
+{
+   // lulesh.cc:166-222 (no MPI reduction: one domain).
+   Real_t targetdt = domain.stoptime() - domain.time();
+   if ((domain.dtfixed() <= 0.0) && (domain.cycle() != 0)) {
+      const Real_t olddt = domain.deltatime();
+      Real_t newdt = 1.0e+20;
+      if (domain.dtcourant() < newdt) newdt = domain.dtcourant() / 2.0;
+      if (domain.dthydro() < newdt) newdt = domain.dthydro() * 2.0 / 3.0;
+      const Real_t ratio = newdt / olddt;
+      if (ratio >= 1.0) {
+         if (ratio < domain.deltatimemultlb()) newdt = olddt;
+         else if (ratio > domain.deltatimemultub()) newdt = olddt * domain.deltatimemultub();
+      }
+      if (newdt > domain.dtmax()) newdt = domain.dtmax();
+      domain.deltatime() = newdt;
+   }
+   if ((targetdt > domain.deltatime()) && (targetdt < (4.0 * domain.deltatime() / 3.0)))
+      targetdt = 2.0 * domain.deltatime() / 3.0;
+   if (targetdt < domain.deltatime()) domain.deltatime() = targetdt;
+   domain.time() += domain.deltatime();
+   ++domain.cycle();
+}
+
+static void exit_on(int st)
+{
+   if (st == LGPU_OK) return;
+   if (st > -100) exit(st);
+   fprintf(stderr, "device path failed (%d)\n", st);
+   exit(st);
+}
+
+void LagrangeNodal(Domain &domain)
+{
+   exit_on(lgpu_lagrange_nodal(domain.device(), domain.deltatime()));
+}
+
+void LagrangeElements(Domain &domain, Index_t)
+{
+   exit_on(lgpu_lagrange_elements(domain.device(), domain.deltatime()));
+}
+
+void CalcTimeConstraintsForElems(Domain &domain)
+{
+   exit_on(lgpu_time_constraints(domain.device(), &domain.dtcourant(), &domain.dthydro()));
+}
+
+void LagrangeLeapFrog(Domain &domain)
+{
+   LagrangeNodal(domain);
+   LagrangeElements(domain, domain.numElem());
+   CalcTimeConstraintsForElems(domain);
+}
+
+int LagrangeLeapFrogStatus(Domain &domain)
+{
+   lgpu_ctx *g = domain.device();
+   if (!g) return LGPU_EINVAL;
+   int st = lgpu_lagrange_nodal(g, domain.deltatime());
+   if (st) return st;
+   st = lgpu_lagrange_elements(g, domain.deltatime());
+   if (st) return st;
+   return lgpu_time_constraints(g, &domain.dtcourant(), &domain.dthydro());
+}
+
+// ---------------------------------------------------------------------------
+// C API.
+
+struct lh_domain {
+   Domain *d;
+};
+
+extern "C" {
+
+lh_domain *lh_create(int32_t s, int32_t nr, int32_t balance, int32_t cost, double emult)
+{
+   if (s < 1 || nr < 1 || cost < 0) return nullptr;
+   lh_domain *h = new lh_domain{new Domain(1, 0, 0, 0, s, 1, nr, balance, cost)};
+   if (emult != 1.0) h->d->ScaleInitialEnergy(emult);
+   return h;
+}
+
+void lh_destroy(lh_domain *h)
+{
+   if (!h) return;
+   delete h->d;
+   delete h;
+}
+
+int lh_attach_device(lh_domain *h, int32_t precision)
+{
+   return h ? h->d->AttachDevice(precision) : LGPU_EINVAL;
+}
+
+lgpu_ctx *lh_device(lh_domain *h) { return h ? h->d->device() : nullptr; }
+
+int lh_step(lh_domain *h)
+{
+   if (!h || !h->d->device()) return LGPU_EINVAL;
+   TimeIncrement(*h->d);
+   return LagrangeLeapFrogStatus(*h->d);
+}
+
+int lh_run(lh_domain *h, int32_t its, int32_t device_loop)
+{
+   if (!h || !h->d->device()) return LGPU_EINVAL;
+   Domain &d = *h->d;
+   if (device_loop) {
+      lgpu_time_state ts;
+      ts.time = d.time();
+      ts.deltatime = d.deltatime();
+      ts.dtcourant = d.dtcourant();
+      ts.dthydro = d.dthydro();
+      ts.stoptime = d.stoptime();
+      ts.dtfixed = d.dtfixed();
+      ts.dtmax = d.dtmax();
+      ts.deltatimemultlb = d.deltatimemultlb();
+      ts.deltatimemultub = d.deltatimemultub();
+      ts.cycle = d.cycle();
+      const int st = lgpu_run(d.device(), &ts, its);
+      d.time() = ts.time;
+      d.deltatime() = ts.deltatime;
+      d.dtcourant() = ts.dtcourant;
+      d.dthydro() = ts.dthydro;
+      d.cycle() = ts.cycle;
+      return st;
+   }
+   while (d.time() < d.stoptime() && d.cycle() < its) {
+      TimeIncrement(d);
+      const int st = LagrangeLeapFrogStatus(d);
+      if (st) return st;
+   }
+   return LGPU_OK;
+}
+
+int lh_sync_from_device(lh_domain *h) { return h ? h->d->SyncFromDevice() : LGPU_EINVAL; }
+int32_t lh_num_elem(lh_domain *h) { return h ? h->d->numElem() : 0; }
+int32_t lh_num_node(lh_domain *h) { return h ? h->d->numNode() : 0; }
+int32_t lh_num_reg(lh_domain *h) { return h ? h->d->numReg() : 0; }
+double *lh_real(lh_domain *h, int32_t f) { return h ? h->d->realField(f) : nullptr; }
+
+int32_t *lh_int(lh_domain *h, int32_t which, int32_t region, int64_t *len)
+{
+   if (!h || !len) return nullptr;
+   Domain &d = *h->d;
+   const int64_t E = d.numElem(), N = d.numNode(), en = d.sizeX() + 1;
+   switch (which) {
+   case 0: *len = 8 * E; return d.nodelist(0);
+   case 1: *len = E; return &d.lxim(0);
+   case 2: *len = E; return &d.lxip(0);
+   case 3: *len = E; return &d.letam(0);
+   case 4: *len = E; return &d.letap(0);
+   case 5: *len = E; return &d.lzetam(0);
+   case 6: *len = E; return &d.lzetap(0);
+   case 7: *len = E; return &d.elemBC(0);
+   case 8: *len = N + 1; return const_cast<Index_t *>(d.nodeElemStartVec().data());
+   case 9: *len = (int64_t)d.cornerListVec().size(); return const_cast<Index_t *>(d.cornerListVec().data());
+   case 10: *len = en * en; return d.symmXempty() ? nullptr : d.symmXdata();
+   case 11: *len = en * en; return d.symmYempty() ? nullptr : d.symmYdata();
+   case 12: *len = en * en; return d.symmZempty() ? nullptr : d.symmZdata();
+   case 13: *len = E; return d.regNumList();
+   case 14: *len = d.numReg(); return &d.regElemSize(0);
+   case 15:
+      if (region < 0 || region >= d.numReg()) break;
+      *len = d.regElemSize(region);
+      return d.regElemlist(region);
+   default: break;
+   }
+   *len = 0;
+   return nullptr;
+}
+
+double lh_get(lh_domain *h, int32_t which)
+{
+   if (!h) return 0.0;
+   Domain &d = *h->d;
+   switch (which) {
+   case 0: return d.time();
+   case 1: return d.deltatime();
+   case 2: return d.dtcourant();
+   case 3: return d.dthydro();
+   case 4: return (double)d.cycle();
+   case 5: return d.stoptime();
+   case 6: return d.dtfixed();
+   case 7: return d.dtmax();
+   case 8: return d.deltatimemultlb();
+   case 9: return d.deltatimemultub();
+   default: return 0.0;
+   }
+}
+
+void lh_set(lh_domain *h, int32_t which, double v)
+{
+   if (!h) return;
+   Domain &d = *h->d;
+   switch (which) {
+   case 0: d.time() = v; break;
+   case 1: d.deltatime() = v; break;
+   case 2: d.dtcourant() = v; break;
+   case 3: d.dthydro() = v; break;
+   case 4: d.cycle() = (Int_t)v; break;
+   case 5: d.stoptime() = v; break;
+   case 6: d.dtfixed() = v; break;
+   case 7: d.dtmax() = v; break;
+   case 8: d.deltatimemultlb() = v; break;
+   case 9: d.deltatimemultub() = v; break;
+   default: break;
+   }
+}
+
+void lh_symmetry(lh_domain *h, double out[3])
+{
+   // lulesh-util.cc:197-212 over the host copy of e.
+   Domain &d = *h->d;
+   const Index_t nx = d.sizeX();
+   Real_t maxAbs = 0.0, tot = 0.0, maxRel = 0.0;
+   for (Index_t j = 0; j < nx; ++j)
+      for (Index_t k = j + 1; k < nx; ++k) {
+         const Real_t ad = fabs(d.e(j * nx + k) - d.e(k * nx + j));
+         tot += ad;
+         if (maxAbs < ad) maxAbs = ad;
+         const Real_t rd = ad / d.e(k * nx + j);
+         if (maxRel < rd) maxRel = rd;
+      }
+   out[0] = maxAbs;
+   out[1] = tot;
+   out[2] = maxRel;
+}
+
+} // extern "C"

@@ -1,0 +1,713 @@
+// lgpu_api.hip -- device context and C ABI (include/lulesh_gpu.h).
+//
+// One context owns one HIP stream and one device arena holding every array
+// the step touches, laid out SoA with 256-byte aligned planes.  All
+// connectivity is validated on the host before the first launch, so no
+// kernel ever indexes outside the arena.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lgpu_kernels.h"
+#include "lgpu_launch.h"
+#include "lulesh_gpu.h"
+
+using namespace lgpu;
+
+namespace {
+
+enum KernelId { kKForce = 0, kKNode, kKKin, kKEos, kKTimeIncrement, kNumKernels };
+const char *const kKernelNames[kNumKernels] = {"k_force", "k_node", "k_kin", "k_eos",
+                                               "k_time_increment"};
+
+constexpr size_t kAlign = 256;
+
+inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+
+struct TimedLaunch {
+   int kernel;
+   hipEvent_t start, stop;
+};
+
+// Device-side time state for lgpu_run: TimeIncrement operands plus the loop
+// flag the kernels test.
+struct DevTime {
+   double time, deltatime, dtcourant, dthydro, stoptime, dtfixed, dtmax, multlb, multub;
+   int32_t cycle, maxCycle, active, pad;
+};
+
+} // namespace
+
+struct lgpu_ctx {
+   int device = 0;
+   hipStream_t stream = nullptr;
+   int32_t numElem = 0, numNode = 0, numReg = 0, cost = 0, precision = 0;
+   int32_t sizeX = 0;
+   int block = 256;
+   bool keepDiag = false;
+   bool timing = false;
+   Consts k{};
+   char *arena = nullptr;
+   size_t arenaBytes = 0;
+   int64_t planeStride = 0; // corner plane length (elements, padded)
+   DevState s{};
+   // pinned readback: dt[2], err[kNumErrSites]
+   double *hostDt = nullptr;
+   int32_t *hostErr = nullptr;
+   DevTime *devTime = nullptr;
+   DevTime *hostTime = nullptr;
+   // diagnostics storage (allocated on first use)
+   double *diag = nullptr;
+   std::vector<TimedLaunch> pending;
+   std::vector<hipEvent_t> eventPool;
+   int64_t kLaunches[kNumKernels] = {};
+   double kMs[kNumKernels] = {};
+   std::string lastError;
+};
+
+namespace {
+
+int set_hip_error(lgpu_ctx *c, hipError_t e, const char *what)
+{
+   if (e == hipSuccess) return LGPU_OK;
+   if (c) c->lastError = std::string(what) + ": " + hipGetErrorString(e);
+   return (e == hipErrorOutOfMemory) ? LGPU_ENOMEM : LGPU_EDEVICE;
+}
+
+#define HIP_OR_RETURN(c, call)                                   \
+   do {                                                          \
+      const hipError_t e_ = (call);                              \
+      if (e_ != hipSuccess) return set_hip_error((c), e_, #call); \
+   } while (0)
+
+hipEvent_t take_event(lgpu_ctx *c)
+{
+   if (!c->eventPool.empty()) {
+      hipEvent_t e = c->eventPool.back();
+      c->eventPool.pop_back();
+      return e;
+   }
+   hipEvent_t e = nullptr;
+   if (hipEventCreate(&e) != hipSuccess) return nullptr;
+   return e;
+}
+
+// Folds finished timed launches into the per-kernel totals.  Call only after
+// the stream has been synchronized.
+void harvest_timing(lgpu_ctx *c)
+{
+   for (TimedLaunch &t : c->pending) {
+      float ms = 0.f;
+      if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+         c->kLaunches[t.kernel] += 1;
+         c->kMs[t.kernel] += ms;
+      }
+      c->eventPool.push_back(t.start);
+      c->eventPool.push_back(t.stop);
+   }
+   c->pending.clear();
+}
+
+template <class F>
+int timed(lgpu_ctx *c, int kernel, F &&launch)
+{
+   TimedLaunch t{kernel, nullptr, nullptr};
+   if (c->timing) {
+      t.start = take_event(c);
+      t.stop = take_event(c);
+      if (t.start && t.stop) HIP_OR_RETURN(c, hipEventRecord(t.start, c->stream));
+   }
+   launch();
+   HIP_OR_RETURN(c, hipGetLastError());
+   if (c->timing && t.start && t.stop) {
+      HIP_OR_RETURN(c, hipEventRecord(t.stop, c->stream));
+      c->pending.push_back(t);
+   }
+   return LGPU_OK;
+}
+
+bool in_range(const int32_t *a, int64_t n, int32_t lo, int32_t hi)
+{
+   for (int64_t i = 0; i < n; ++i)
+      if (a[i] < lo || a[i] >= hi) return false;
+   return true;
+}
+
+int validate(const lgpu_mesh_desc *d, std::string &why)
+{
+   if (!d) { why = "null description"; return LGPU_EINVAL; }
+   const int64_t E = d->numElem, N = d->numNode;
+   if (E < 1 || N < 1 || E > (INT32_MAX / 8)) { why = "numElem/numNode out of range"; return LGPU_EINVAL; }
+   if (!d->nodelist || !d->lxim || !d->lxip || !d->letam || !d->letap || !d->lzetam ||
+       !d->lzetap || !d->elemBC || !d->nodeElemStart || !d->nodeElemCornerList ||
+       !d->regElemSize || !d->regElemlist) {
+      why = "missing array";
+      return LGPU_EINVAL;
+   }
+   if (d->precision != LGPU_FP64 && d->precision != LGPU_MIXED) { why = "precision"; return LGPU_EINVAL; }
+   if (d->numReg < 1 || d->cost < 0) { why = "numReg/cost"; return LGPU_EINVAL; }
+   if (!in_range(d->nodelist, 8 * E, 0, (int32_t)N)) { why = "nodelist index out of range"; return LGPU_EINVAL; }
+   const int32_t *nb[6] = {d->lxim, d->lxip, d->letam, d->letap, d->lzetam, d->lzetap};
+   for (const int32_t *a : nb)
+      if (!in_range(a, E, 0, (int32_t)E)) { why = "face neighbour out of range"; return LGPU_EINVAL; }
+   if (d->nodeElemStart[0] != 0) { why = "nodeElemStart[0] != 0"; return LGPU_EINVAL; }
+   for (int64_t n = 0; n < N; ++n)
+      if (d->nodeElemStart[n + 1] < d->nodeElemStart[n]) { why = "nodeElemStart not monotone"; return LGPU_EINVAL; }
+   const int64_t nc = d->nodeElemStart[N];
+   if (nc > 8 * E) { why = "corner list too long"; return LGPU_EINVAL; }
+   if (!in_range(d->nodeElemCornerList, nc, 0, (int32_t)(8 * E))) { why = "corner index out of range"; return LGPU_EINVAL; }
+   const int32_t *sy[3] = {d->symmX, d->symmY, d->symmZ};
+   const int32_t ns[3] = {d->numSymmX, d->numSymmY, d->numSymmZ};
+   for (int a = 0; a < 3; ++a) {
+      if (ns[a] < 0 || (ns[a] > 0 && !sy[a])) { why = "symmetry set"; return LGPU_EINVAL; }
+      if (ns[a] > 0 && !in_range(sy[a], ns[a], 0, (int32_t)N)) { why = "symmetry node out of range"; return LGPU_EINVAL; }
+   }
+   // regions must partition the elements
+   std::vector<uint8_t> seen((size_t)E, 0);
+   int64_t total = 0;
+   for (int r = 0; r < d->numReg; ++r) {
+      const int32_t len = d->regElemSize[r];
+      if (len < 0 || (len > 0 && !d->regElemlist[r])) { why = "region list"; return LGPU_EINVAL; }
+      for (int32_t i = 0; i < len; ++i) {
+         const int32_t e = d->regElemlist[r][i];
+         if (e < 0 || e >= E || seen[(size_t)e]) { why = "regions do not partition the elements"; return LGPU_EINVAL; }
+         seen[(size_t)e] = 1;
+      }
+      total += len;
+   }
+   if (total != E) { why = "regions do not cover every element"; return LGPU_EINVAL; }
+   return LGPU_OK;
+}
+
+// EOS repeat count of region r, lulesh.cc:2390-2400.
+int32_t region_rep(int32_t r, int32_t numReg, int32_t cost)
+{
+   if (r < numReg / 2) return 1;
+   if (r < numReg - (numReg + 15) / 20) return 1 + cost;
+   return 10 * (1 + cost);
+}
+
+double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &isGrad)
+{
+   isElem = f >= LGPU_E;
+   isGrad = f >= LGPU_DELX_XI;
+   DevState &s = c->s;
+   switch (f) {
+   case LGPU_X: return s.x;
+   case LGPU_Y: return s.y;
+   case LGPU_Z: return s.z;
+   case LGPU_XD: return s.xd;
+   case LGPU_YD: return s.yd;
+   case LGPU_ZD: return s.zd;
+   case LGPU_XDD: return s.xdd;
+   case LGPU_YDD: return s.ydd;
+   case LGPU_ZDD: return s.zdd;
+   case LGPU_FX: return s.fx;
+   case LGPU_FY: return s.fy;
+   case LGPU_FZ: return s.fz;
+   case LGPU_NODALMASS: return s.nodalMass;
+   case LGPU_E: return s.e;
+   case LGPU_P: return s.p;
+   case LGPU_Q: return s.q;
+   case LGPU_QL: return s.ql;
+   case LGPU_QQ: return s.qq;
+   case LGPU_V: return s.v;
+   case LGPU_VOLO: return s.volo;
+   case LGPU_VNEW: return s.vnew;
+   case LGPU_DELV: return s.delv;
+   case LGPU_VDOV: return s.vdov;
+   case LGPU_AREALG: return s.arealg;
+   case LGPU_SS: return s.ss;
+   case LGPU_ELEMMASS: return s.elemMass;
+   case LGPU_DELX_XI: return static_cast<double *>(s.delx_xi);
+   case LGPU_DELX_ETA: return static_cast<double *>(s.delx_eta);
+   case LGPU_DELX_ZETA: return static_cast<double *>(s.delx_zeta);
+   case LGPU_DELV_XI: return static_cast<double *>(s.delv_xi);
+   case LGPU_DELV_ETA: return static_cast<double *>(s.delv_eta);
+   case LGPU_DELV_ZETA: return static_cast<double *>(s.delv_zeta);
+   default: return nullptr;
+   }
+}
+
+int enable_diagnostics(lgpu_ctx *c)
+{
+   if (c->diag) return LGPU_OK;
+   const size_t nn = align_up(sizeof(double) * (size_t)c->numNode);
+   const size_t ne = align_up(sizeof(double) * (size_t)c->numElem);
+   char *p = nullptr;
+   HIP_OR_RETURN(c, hipMalloc((void **)&p, 6 * nn + 2 * ne));
+   HIP_OR_RETURN(c, hipMemset(p, 0, 6 * nn + 2 * ne));
+   c->diag = reinterpret_cast<double *>(p);
+   double **node[6] = {&c->s.fx, &c->s.fy, &c->s.fz, &c->s.xdd, &c->s.ydd, &c->s.zdd};
+   for (int i = 0; i < 6; ++i) *node[i] = reinterpret_cast<double *>(p + i * nn);
+   c->s.ql = reinterpret_cast<double *>(p + 6 * nn);
+   c->s.qq = reinterpret_cast<double *>(p + 6 * nn + ne);
+   return LGPU_OK;
+}
+
+template <class P>
+int launch_nodal(lgpu_ctx *c, double dt)
+{
+   int r = timed(c, kKForce, [&] { Launch<P>::force(c->s, c->k, c->planeStride, c->block, c->stream); });
+   if (r) return r;
+   return timed(c, kKNode, [&] { Launch<P>::node(c->s, c->k, c->planeStride, dt, c->block, c->stream); });
+}
+
+template <class P>
+int launch_elements(lgpu_ctx *c, double dt)
+{
+   int r = timed(c, kKKin, [&] { Launch<P>::kin(c->s, c->k, dt, c->block, c->stream); });
+   if (r) return r;
+   return timed(c, kKEos, [&] { Launch<P>::eos(c->s, c->k, c->stream); });
+}
+
+int resolve_status(const int32_t *err)
+{
+   if (err[kErrForceDeterm] || err[kErrHourglassV] || err[kErrKinVnew]) return LGPU_VOLUME_ERROR;
+   if (err[kErrQStop]) return LGPU_QSTOP_ERROR;
+   if (err[kErrMaterialV]) return LGPU_VOLUME_ERROR;
+   return LGPU_OK;
+}
+
+// TimeIncrement (lulesh.cc:166-222) for the device-side loop; also decides
+// whether this cycle runs at all (the loop test of lulesh.cc:2745).
+__global__ void k_time_increment(DevTime *t, const double *dtOut, const int32_t *err)
+{
+   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+   if (!t->active) return;
+   bool failed = false;
+   for (int i = 0; i < kNumErrSites; ++i) failed = failed || (err[i] != 0);
+   if (t->cycle > 0) {
+      t->dtcourant = dtOut[0];
+      t->dthydro = dtOut[1];
+   }
+   if (failed || !(t->time < t->stoptime && t->cycle < t->maxCycle)) {
+      t->active = 0;
+      return;
+   }
+   double targetdt = t->stoptime - t->time;
+   if (t->dtfixed <= 0.0 && t->cycle != 0) {
+      const double olddt = t->deltatime;
+      double newdt = 1.0e+20;
+      if (t->dtcourant < newdt) newdt = t->dtcourant / 2.0;
+      if (t->dthydro < newdt) newdt = t->dthydro * 2.0 / 3.0;
+      const double ratio = newdt / olddt;
+      if (ratio >= 1.0) {
+         if (ratio < t->multlb) newdt = olddt;
+         else if (ratio > t->multub) newdt = olddt * t->multub;
+      }
+      if (newdt > t->dtmax) newdt = t->dtmax;
+      t->deltatime = newdt;
+   }
+   if (targetdt > t->deltatime && targetdt < 4.0 * t->deltatime / 3.0)
+      targetdt = 2.0 * t->deltatime / 3.0;
+   if (targetdt < t->deltatime) t->deltatime = targetdt;
+   t->time += t->deltatime;
+   t->cycle += 1;
+}
+
+// Folds the last cycle's dt results into the time state once the loop ends.
+__global__ void k_time_final(DevTime *t, const double *dtOut)
+{
+   if (threadIdx.x != 0 || blockIdx.x != 0) return;
+   if (t->cycle > 0) {
+      t->dtcourant = dtOut[0];
+      t->dthydro = dtOut[1];
+   }
+}
+
+} // namespace
+
+extern "C" {
+
+void lgpu_default_constants(lgpu_constants *k)
+{
+   if (!k) return;
+   k->e_cut = 1.0e-7;
+   k->p_cut = 1.0e-7;
+   k->q_cut = 1.0e-7;
+   k->v_cut = 1.0e-10;
+   k->u_cut = 1.0e-7;
+   k->hgcoef = 3.0;
+   k->ss4o3 = 4.0 / 3.0;
+   k->qstop = 1.0e+12;
+   k->monoq_max_slope = 1.0;
+   k->monoq_limiter_mult = 2.0;
+   k->qlc_monoq = 0.5;
+   k->qqc_monoq = 2.0 / 3.0;
+   k->qqc = 2.0;
+   k->eosvmax = 1.0e+9;
+   k->eosvmin = 1.0e-9;
+   k->pmin = 0.0;
+   k->emin = -1.0e+15;
+   k->dvovmax = 0.1;
+   k->refdens = 1.0;
+}
+
+int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
+{
+   if (!out) return LGPU_EINVAL;
+   *out = nullptr;
+   std::string why;
+   if (validate(d, why) != LGPU_OK) {
+      fprintf(stderr, "lgpu_create: invalid mesh description: %s\n", why.c_str());
+      return LGPU_EINVAL;
+   }
+   lgpu_ctx *c = new (std::nothrow) lgpu_ctx();
+   if (!c) return LGPU_ENOMEM;
+   auto fail = [&](int code) {
+      lgpu_destroy(c);
+      return code;
+   };
+   int dev = 0;
+   if (hipGetDevice(&dev) != hipSuccess) return fail(LGPU_EDEVICE);
+   c->device = dev;
+   c->numElem = d->numElem;
+   c->numNode = d->numNode;
+   c->numReg = d->numReg;
+   c->cost = d->cost;
+   c->precision = d->precision;
+   c->sizeX = d->sizeX;
+   const lgpu_constants &kk = d->k;
+   c->k = Consts{kk.e_cut, kk.p_cut, kk.q_cut, kk.v_cut, kk.u_cut, kk.hgcoef, kk.ss4o3,
+                 kk.qstop, kk.monoq_max_slope, kk.monoq_limiter_mult, kk.qlc_monoq,
+                 kk.qqc_monoq, kk.qqc, kk.eosvmax, kk.eosvmin, kk.pmin, kk.emin, kk.dvovmax,
+                 kk.refdens};
+   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+      return fail(LGPU_EDEVICE);
+
+   const size_t E = (size_t)d->numElem, N = (size_t)d->numNode;
+   const size_t gradBytes = (d->precision == LGPU_MIXED) ? 4 : 8;
+   const size_t cornerBytes = (d->precision == LGPU_MIXED) ? 4 : 8;
+   const int64_t plane = (int64_t)(align_up(E * cornerBytes) / cornerBytes);
+   c->planeStride = plane;
+   const int64_t nCorner = d->nodeElemStart[N];
+   const unsigned eosBlocks = eos_blocks(d->numElem);
+
+   // Arena layout.
+   struct Slot { void **dst; size_t bytes; };
+   std::vector<Slot> slots;
+   DevState &s = c->s;
+   auto add = [&](void **dst, size_t bytes) { slots.push_back({dst, align_up(bytes)}); };
+   double **nodeReal[] = {&s.x, &s.y, &s.z, &s.xd, &s.yd, &s.zd, &s.nodalMass};
+   for (double **p : nodeReal) add((void **)p, sizeof(double) * N);
+   add((void **)&s.nodeBC, N);
+   add((void **)&s.nodeElemStart, sizeof(int32_t) * (N + 1));
+   add((void **)&s.cornerList, sizeof(int32_t) * (size_t)std::max<int64_t>(nCorner, 1));
+   add((void **)&s.nodelist, sizeof(int32_t) * 8 * E);
+   const int32_t **nbr[] = {&s.lxim, &s.lxip, &s.letam, &s.letap, &s.lzetam, &s.lzetap, &s.elemBC};
+   for (const int32_t **p : nbr) add((void **)p, sizeof(int32_t) * E);
+   double **elemReal[] = {&s.e, &s.p, &s.q, &s.v, &s.volo, &s.ss, &s.elemMass,
+                          &s.vnew, &s.delv, &s.arealg, &s.vdov};
+   for (double **p : elemReal) add((void **)p, sizeof(double) * E);
+   void **grads[] = {&s.delx_xi, &s.delx_eta, &s.delx_zeta, &s.delv_xi, &s.delv_eta, &s.delv_zeta};
+   for (void **p : grads) add(p, gradBytes * E);
+   add(&s.corner, cornerBytes * (size_t)plane * 48);
+   add((void **)&s.perm, sizeof(int32_t) * E);
+   add((void **)&s.regStart, sizeof(int32_t) * (size_t)(d->numReg + 1));
+   add((void **)&s.regRep, sizeof(int32_t) * (size_t)d->numReg);
+   add((void **)&s.err, sizeof(int32_t) * kNumErrSites);
+   add((void **)&s.dtPartial, sizeof(double) * 2 * eosBlocks);
+   add((void **)&s.dtCounter, sizeof(uint32_t));
+   add((void **)&s.dtOut, sizeof(double) * 2);
+   add((void **)&c->devTime, sizeof(DevTime));
+   size_t total = 0;
+   for (const Slot &sl : slots) total += sl.bytes;
+   if (hipMalloc((void **)&c->arena, total) != hipSuccess) {
+      (void)hipGetLastError();
+      fprintf(stderr, "lgpu_create: device arena of %zu bytes failed\n", total);
+      return fail(LGPU_ENOMEM);
+   }
+   c->arenaBytes = total;
+   if (hipMemset(c->arena, 0, total) != hipSuccess) return fail(LGPU_EDEVICE);
+   size_t off = 0;
+   for (const Slot &sl : slots) {
+      *sl.dst = c->arena + off;
+      off += sl.bytes;
+   }
+   s.numElem = d->numElem;
+   s.numNode = d->numNode;
+   s.numReg = d->numReg;
+   s.xcdRemap = 1;
+
+   // Host-built tables.
+   std::vector<uint8_t> bc(N, 0);
+   const int32_t *sy[3] = {d->symmX, d->symmY, d->symmZ};
+   const int32_t ns[3] = {d->numSymmX, d->numSymmY, d->numSymmZ};
+   for (int a = 0; a < 3; ++a)
+      for (int32_t i = 0; i < ns[a]; ++i) bc[(size_t)sy[a][i]] |= (uint8_t)(1u << a);
+   std::vector<int32_t> perm;
+   perm.reserve(E);
+   std::vector<int32_t> regStart((size_t)d->numReg + 1, 0), regRep((size_t)d->numReg, 1);
+   for (int r = 0; r < d->numReg; ++r) {
+      regStart[(size_t)r] = (int32_t)perm.size();
+      perm.insert(perm.end(), d->regElemlist[r], d->regElemlist[r] + d->regElemSize[r]);
+      regRep[(size_t)r] = region_rep(r, d->numReg, d->cost);
+   }
+   regStart[(size_t)d->numReg] = (int32_t)perm.size();
+   // Empty regions sit at the same offset as the next one; the binary search
+   // in k_eos picks the last region starting at or before a position, which
+   // is the non-empty one holding it.
+   std::vector<double> ones(E, 1.0);
+
+   auto h2d = [&](const void *dst, const void *src, size_t bytes) {
+      return hipMemcpy(const_cast<void *>(dst), src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+   };
+   bool ok = h2d(s.nodeBC, bc.data(), N) &&
+             h2d(s.nodeElemStart, d->nodeElemStart, sizeof(int32_t) * (N + 1)) &&
+             (nCorner == 0 || h2d(s.cornerList, d->nodeElemCornerList, sizeof(int32_t) * (size_t)nCorner)) &&
+             h2d(s.nodelist, d->nodelist, sizeof(int32_t) * 8 * E) &&
+             h2d(s.lxim, d->lxim, sizeof(int32_t) * E) && h2d(s.lxip, d->lxip, sizeof(int32_t) * E) &&
+             h2d(s.letam, d->letam, sizeof(int32_t) * E) && h2d(s.letap, d->letap, sizeof(int32_t) * E) &&
+             h2d(s.lzetam, d->lzetam, sizeof(int32_t) * E) && h2d(s.lzetap, d->lzetap, sizeof(int32_t) * E) &&
+             h2d(s.elemBC, d->elemBC, sizeof(int32_t) * E) &&
+             h2d(s.perm, perm.data(), sizeof(int32_t) * E) &&
+             h2d(s.regStart, regStart.data(), sizeof(int32_t) * regStart.size()) &&
+             h2d(s.regRep, regRep.data(), sizeof(int32_t) * regRep.size()) &&
+             h2d(s.v, ones.data(), sizeof(double) * E);
+   if (!ok) return fail(LGPU_EDEVICE);
+
+   if (hipHostMalloc((void **)&c->hostDt, sizeof(double) * 2 + sizeof(int32_t) * kNumErrSites +
+                                               sizeof(DevTime) + 64, hipHostMallocDefault) != hipSuccess)
+      return fail(LGPU_ENOMEM);
+   c->hostErr = reinterpret_cast<int32_t *>(c->hostDt + 2);
+   c->hostTime = reinterpret_cast<DevTime *>(reinterpret_cast<char *>(c->hostDt) + 64);
+   *out = c;
+   return LGPU_OK;
+}
+
+void lgpu_destroy(lgpu_ctx *c)
+{
+   if (!c) return;
+   if (c->stream) (void)hipStreamSynchronize(c->stream);
+   for (TimedLaunch &t : c->pending) {
+      (void)hipEventDestroy(t.start);
+      (void)hipEventDestroy(t.stop);
+   }
+   for (hipEvent_t e : c->eventPool) (void)hipEventDestroy(e);
+   if (c->diag) (void)hipFree(c->diag);
+   if (c->arena) (void)hipFree(c->arena);
+   if (c->hostDt) (void)hipHostFree(c->hostDt);
+   if (c->stream) (void)hipStreamDestroy(c->stream);
+   delete c;
+}
+
+int lgpu_upload(lgpu_ctx *c, int field, const double *host)
+{
+   if (!c || !host || field < 0 || field >= LGPU_NUM_FIELDS) return LGPU_EINVAL;
+   bool isElem, isGrad;
+   if ((field >= LGPU_XDD && field <= LGPU_FZ) || field == LGPU_QL || field == LGPU_QQ) {
+      int r = enable_diagnostics(c);
+      if (r) return r;
+   }
+   double *dst = real_field_ptr(c, field, isElem, isGrad);
+   if (!dst) return LGPU_EINVAL;
+   const size_t n = isElem ? (size_t)c->numElem : (size_t)c->numNode;
+   HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
+   if (isGrad && c->precision == LGPU_MIXED) {
+      std::vector<float> tmp(n);
+      for (size_t i = 0; i < n; ++i) tmp[i] = (float)host[i];
+      HIP_OR_RETURN(c, hipMemcpy(dst, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice));
+   } else {
+      HIP_OR_RETURN(c, hipMemcpy(dst, host, sizeof(double) * n, hipMemcpyHostToDevice));
+   }
+   return LGPU_OK;
+}
+
+int lgpu_download(lgpu_ctx *c, int field, double *host)
+{
+   if (!c || !host || field < 0 || field >= LGPU_NUM_FIELDS) return LGPU_EINVAL;
+   bool isElem, isGrad;
+   double *src = real_field_ptr(c, field, isElem, isGrad);
+   const size_t n = isElem ? (size_t)c->numElem : (size_t)c->numNode;
+   HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
+   if (!src) {
+      std::fill(host, host + n, 0.0);
+      return LGPU_OK;
+   }
+   if (isGrad && c->precision == LGPU_MIXED) {
+      std::vector<float> tmp(n);
+      HIP_OR_RETURN(c, hipMemcpy(tmp.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n; ++i) host[i] = (double)tmp[i];
+   } else {
+      HIP_OR_RETURN(c, hipMemcpy(host, src, sizeof(double) * n, hipMemcpyDeviceToHost));
+   }
+   return LGPU_OK;
+}
+
+int lgpu_lagrange_nodal(lgpu_ctx *c, double dt)
+{
+   if (!c) return LGPU_EINVAL;
+   return (c->precision == LGPU_MIXED) ? launch_nodal<PolicyMixed>(c, dt)
+                                       : launch_nodal<PolicyFp64>(c, dt);
+}
+
+int lgpu_lagrange_elements(lgpu_ctx *c, double dt)
+{
+   if (!c) return LGPU_EINVAL;
+   return (c->precision == LGPU_MIXED) ? launch_elements<PolicyMixed>(c, dt)
+                                       : launch_elements<PolicyFp64>(c, dt);
+}
+
+int lgpu_time_constraints(lgpu_ctx *c, double *dtcourant, double *dthydro)
+{
+   if (!c) return LGPU_EINVAL;
+   HIP_OR_RETURN(c, hipMemcpyAsync(c->hostDt, c->s.dtOut, sizeof(double) * 2,
+                                   hipMemcpyDeviceToHost, c->stream));
+   HIP_OR_RETURN(c, hipMemcpyAsync(c->hostErr, c->s.err, sizeof(int32_t) * kNumErrSites,
+                                   hipMemcpyDeviceToHost, c->stream));
+   HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
+   if (c->timing) harvest_timing(c);
+   const int st = resolve_status(c->hostErr);
+   if (st == LGPU_OK) {
+      if (dtcourant) *dtcourant = c->hostDt[0];
+      if (dthydro) *dthydro = c->hostDt[1];
+   }
+   return st;
+}
+
+int lgpu_run(lgpu_ctx *c, lgpu_time_state *ts, int32_t maxCycle)
+{
+   if (!c || !ts) return LGPU_EINVAL;
+   DevTime *h = c->hostTime;
+   h->time = ts->time;
+   h->deltatime = ts->deltatime;
+   h->dtcourant = ts->dtcourant;
+   h->dthydro = ts->dthydro;
+   h->stoptime = ts->stoptime;
+   h->dtfixed = ts->dtfixed;
+   h->dtmax = ts->dtmax;
+   h->multlb = ts->deltatimemultlb;
+   h->multub = ts->deltatimemultub;
+   h->cycle = ts->cycle;
+   h->maxCycle = maxCycle;
+   h->active = 1;
+   h->pad = 0;
+   HIP_OR_RETURN(c, hipMemcpyAsync(c->devTime, h, sizeof(DevTime), hipMemcpyHostToDevice, c->stream));
+   // dtOut feeds the first TimeIncrement only when cycle > 0: seed it with
+   // the caller's constraints.
+   c->hostDt[0] = ts->dtcourant;
+   c->hostDt[1] = ts->dthydro;
+   HIP_OR_RETURN(c, hipMemcpyAsync(c->s.dtOut, c->hostDt, sizeof(double) * 2,
+                                   hipMemcpyHostToDevice, c->stream));
+   HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
+
+   DevState saved = c->s;
+   c->s.dtDevice = &c->devTime->deltatime;
+   c->s.active = &c->devTime->active;
+   const int batch = 16;
+   int st = LGPU_OK;
+   for (;;) {
+      for (int b = 0; b < batch; ++b) {
+         st = timed(c, kKTimeIncrement, [&] {
+            hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, c->stream, c->devTime,
+                               c->s.dtOut, c->s.err);
+         });
+         if (st) break;
+         st = lgpu_lagrange_nodal(c, 0.0);
+         if (st) break;
+         st = lgpu_lagrange_elements(c, 0.0);
+         if (st) break;
+      }
+      if (st) break;
+      HIP_OR_RETURN(c, hipMemcpyAsync(h, c->devTime, sizeof(DevTime), hipMemcpyDeviceToHost, c->stream));
+      HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
+      if (c->timing) harvest_timing(c);
+      if (!h->active) break;
+   }
+   c->s = saved;
+   if (st) return st;
+   hipLaunchKernelGGL(k_time_final, dim3(1), dim3(64), 0, c->stream, c->devTime, c->s.dtOut);
+   HIP_OR_RETURN(c, hipGetLastError());
+   HIP_OR_RETURN(c, hipMemcpyAsync(h, c->devTime, sizeof(DevTime), hipMemcpyDeviceToHost, c->stream));
+   HIP_OR_RETURN(c, hipMemcpyAsync(c->hostErr, c->s.err, sizeof(int32_t) * kNumErrSites,
+                                   hipMemcpyDeviceToHost, c->stream));
+   HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
+   ts->time = h->time;
+   ts->deltatime = h->deltatime;
+   ts->dtcourant = h->dtcourant;
+   ts->dthydro = h->dthydro;
+   ts->cycle = h->cycle;
+   return resolve_status(c->hostErr);
+}
+
+int lgpu_set_option(lgpu_ctx *c, int option, int64_t value)
+{
+   if (!c) return LGPU_EINVAL;
+   switch (option) {
+   case LGPU_OPT_KEEP_DIAGNOSTICS:
+      if (value) {
+         const int r = enable_diagnostics(c);
+         if (r) return r;
+      } else {
+         c->s.fx = c->s.fy = c->s.fz = c->s.xdd = c->s.ydd = c->s.zdd = nullptr;
+         c->s.ql = c->s.qq = nullptr;
+         HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
+         if (c->diag) (void)hipFree(c->diag);
+         c->diag = nullptr;
+      }
+      c->keepDiag = value != 0;
+      return LGPU_OK;
+   case LGPU_OPT_KERNEL_TIMING:
+      c->timing = value != 0;
+      return LGPU_OK;
+   case LGPU_OPT_BLOCK_SIZE:
+      if (value != 64 && value != 128 && value != 256) return LGPU_EINVAL;
+      c->block = (int)value;
+      return LGPU_OK;
+   default:
+      return LGPU_EINVAL;
+   }
+}
+
+int lgpu_num_kernels(void) { return kNumKernels; }
+
+const char *lgpu_kernel_name(int kernel)
+{
+   return (kernel >= 0 && kernel < kNumKernels) ? kKernelNames[kernel] : "";
+}
+
+int lgpu_kernel_stats(lgpu_ctx *c, int kernel, int64_t *launches, double *total_ms)
+{
+   if (!c || kernel < 0 || kernel >= kNumKernels) return LGPU_EINVAL;
+   HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
+   harvest_timing(c);
+   if (launches) *launches = c->kLaunches[kernel];
+   if (total_ms) *total_ms = c->kMs[kernel];
+   return LGPU_OK;
+}
+
+int lgpu_reset_kernel_stats(lgpu_ctx *c)
+{
+   if (!c) return LGPU_EINVAL;
+   HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
+   harvest_timing(c);
+   for (int i = 0; i < kNumKernels; ++i) {
+      c->kLaunches[i] = 0;
+      c->kMs[i] = 0.0;
+   }
+   return LGPU_OK;
+}
+
+int lgpu_get_info(lgpu_ctx *c, lgpu_info *info)
+{
+   if (!c || !info) return LGPU_EINVAL;
+   info->path = 0;
+   info->precision = c->precision;
+   info->numElem = c->numElem;
+   info->numNode = c->numNode;
+   info->numReg = c->numReg;
+   info->block = c->block;
+   info->arena_bytes = (int64_t)c->arenaBytes;
+   return LGPU_OK;
+}
+
+const char *lgpu_last_error(lgpu_ctx *c) { return c ? c->lastError.c_str() : ""; }
+
+} // extern "C"

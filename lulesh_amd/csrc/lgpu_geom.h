@@ -1,0 +1,231 @@
+// lgpu_geom.h -- per-element geometry of the hexahedral Lagrange step, as
+// register-resident device functions.
+//
+// Every function keeps the operand association of the reference line it
+// cites, so that the FP64 instantiation (built without FMA contraction)
+// rounds exactly as the reference does.  T is double for the FP64 path and
+// float for the FP32 parts of the mixed path.
+//
+// Hex node numbering (lulesh-init.cc:252-259): 0 (i,j,k), 1 (i+1,j,k),
+// 2 (i+1,j+1,k), 3 (i,j+1,k), 4..7 the same at k+1.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace lgpu {
+
+#define LG_DEV __device__ __forceinline__
+
+// u . (v x w); the triple product of lulesh.cc:1337-1338 as it is applied
+// at lulesh.cc:1341-1349.
+template <typename T>
+LG_DEV T triple(T ux, T uy, T uz, T vx, T vy, T vz, T wx, T wy, T wz)
+{
+   return ux * (vy * wz - vz * wy) + uy * (vz * wx - vx * wz) + uz * (vx * wy - vy * wx);
+}
+
+// Hexahedron volume from its node coordinates, lulesh.cc:1273-1356.
+template <typename T>
+LG_DEV T hex_volume(const T (&x)[8], const T (&y)[8], const T (&z)[8])
+{
+   const T t1 = triple<T>((x[3] - x[1]) + (x[7] - x[2]), (y[3] - y[1]) + (y[7] - y[2]),
+                          (z[3] - z[1]) + (z[7] - z[2]),
+                          x[6] - x[3], y[6] - y[3], z[6] - z[3],
+                          x[2] - x[0], y[2] - y[0], z[2] - z[0]);
+   const T t2 = triple<T>((x[4] - x[3]) + (x[5] - x[7]), (y[4] - y[3]) + (y[5] - y[7]),
+                          (z[4] - z[3]) + (z[5] - z[7]),
+                          x[6] - x[4], y[6] - y[4], z[6] - z[4],
+                          x[7] - x[0], y[7] - y[0], z[7] - z[0]);
+   const T t3 = triple<T>((x[1] - x[4]) + (x[2] - x[5]), (y[1] - y[4]) + (y[2] - y[5]),
+                          (z[1] - z[4]) + (z[2] - z[5]),
+                          x[6] - x[1], y[6] - y[1], z[6] - z[1],
+                          x[5] - x[0], y[5] - y[0], z[5] - z[0]);
+   T v = t1 + t2 + t3;
+   v *= T(1.0) / T(12.0);
+   return v;
+}
+
+// Centre Jacobian of the trilinear map, one physical coordinate (the three
+// entries of lulesh.cc:319-329 for that coordinate).
+template <typename T>
+struct JacRow {
+   T xi, eta, zeta;
+};
+
+template <typename T>
+LG_DEV JacRow<T> jac_row(const T (&c)[8])
+{
+   const T a = c[6] - c[0], b = c[5] - c[3], g = c[7] - c[1], h = c[4] - c[2];
+   JacRow<T> r;
+   r.xi = T(.125) * (a + b - g - h);
+   r.eta = T(.125) * (a - b + g - h);
+   r.zeta = T(.125) * (a + b + g + h);
+   return r;
+}
+
+// Jacobian determinant ("volume") of CalcElemShapeFunctionDerivatives,
+// lulesh.cc:332-376, without the derivative rows (IntegrateStressForElems
+// overwrites them with face normals, lulesh.cc:534-538).
+template <typename T>
+LG_DEV T jacobian_volume(const T (&x)[8], const T (&y)[8], const T (&z)[8])
+{
+   const JacRow<T> jx = jac_row(x), jy = jac_row(y), jz = jac_row(z);
+   const T cxet = (jz.xi * jy.zeta) - (jy.xi * jz.zeta);
+   const T cyet = (jx.xi * jz.zeta) - (jz.xi * jx.zeta);
+   const T czet = (jy.xi * jx.zeta) - (jx.xi * jy.zeta);
+   return T(8.) * (jx.eta * cxet + jy.eta * cyet + jz.eta * czet);
+}
+
+// Shape-function derivatives of nodes 0..3 (4..7 are their negatives,
+// lulesh.cc:344-373) and the Jacobian determinant, lulesh.cc:290-377.
+template <typename T>
+LG_DEV T shape_derivatives(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&b)[3][4])
+{
+   const JacRow<T> jx = jac_row(x), jy = jac_row(y), jz = jac_row(z);
+   const T c[3][3] = {
+      {(jy.eta * jz.zeta) - (jz.eta * jy.zeta), (jz.xi * jy.zeta) - (jy.xi * jz.zeta),
+       (jy.xi * jz.eta) - (jz.xi * jy.eta)},
+      {(jz.eta * jx.zeta) - (jx.eta * jz.zeta), (jx.xi * jz.zeta) - (jz.xi * jx.zeta),
+       (jz.xi * jx.eta) - (jx.xi * jz.eta)},
+      {(jx.eta * jy.zeta) - (jy.eta * jx.zeta), (jy.xi * jx.zeta) - (jx.xi * jy.zeta),
+       (jx.xi * jy.eta) - (jy.xi * jx.eta)}};
+#pragma unroll
+   for (int r = 0; r < 3; ++r) {
+      b[r][0] = -c[r][0] - c[r][1] - c[r][2];
+      b[r][1] = c[r][0] - c[r][1] - c[r][2];
+      b[r][2] = c[r][0] + c[r][1] - c[r][2];
+      b[r][3] = -c[r][0] + c[r][1] - c[r][2];
+   }
+   return T(8.) * (jx.eta * c[0][1] + jy.eta * c[1][1] + jz.eta * c[2][1]);
+}
+
+// Area-weighted normal of one quadrilateral face (n0,n1,n2,n3), added to its
+// four nodes: SumElemFaceNormal, lulesh.cc:381-415.
+template <typename T, int N0, int N1, int N2, int N3>
+LG_DEV void add_face_normal(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&nx)[8],
+                            T (&ny)[8], T (&nz)[8])
+{
+   const T b0x = T(0.5) * (x[N3] + x[N2] - x[N1] - x[N0]);
+   const T b0y = T(0.5) * (y[N3] + y[N2] - y[N1] - y[N0]);
+   const T b0z = T(0.5) * (z[N3] + z[N2] - z[N1] - z[N0]);
+   const T b1x = T(0.5) * (x[N2] + x[N1] - x[N3] - x[N0]);
+   const T b1y = T(0.5) * (y[N2] + y[N1] - y[N3] - y[N0]);
+   const T b1z = T(0.5) * (z[N2] + z[N1] - z[N3] - z[N0]);
+   const T ax = T(0.25) * (b0y * b1z - b0z * b1y);
+   const T ay = T(0.25) * (b0z * b1x - b0x * b1z);
+   const T az = T(0.25) * (b0x * b1y - b0y * b1x);
+   nx[N0] += ax; nx[N1] += ax; nx[N2] += ax; nx[N3] += ax;
+   ny[N0] += ay; ny[N1] += ay; ny[N2] += ay; ny[N3] += ay;
+   nz[N0] += az; nz[N1] += az; nz[N2] += az; nz[N3] += az;
+}
+
+// CalcElemNodeNormals, lulesh.cc:419-474 (faces in the reference's order, so
+// each node's three contributions are summed in the same order).
+template <typename T>
+LG_DEV void node_normals(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&nx)[8],
+                         T (&ny)[8], T (&nz)[8])
+{
+#pragma unroll
+   for (int i = 0; i < 8; ++i) nx[i] = ny[i] = nz[i] = T(0);
+   add_face_normal<T, 0, 1, 2, 3>(x, y, z, nx, ny, nz);
+   add_face_normal<T, 0, 4, 5, 1>(x, y, z, nx, ny, nz);
+   add_face_normal<T, 1, 5, 6, 2>(x, y, z, nx, ny, nz);
+   add_face_normal<T, 2, 6, 7, 3>(x, y, z, nx, ny, nz);
+   add_face_normal<T, 3, 7, 4, 0>(x, y, z, nx, ny, nz);
+   add_face_normal<T, 4, 7, 6, 5>(x, y, z, nx, ny, nz);
+}
+
+// Bilinear six-node form of VoluDer (lulesh.cc:602-614) for one node whose
+// six neighbours are (A0..A5): dV/dx = P(y,z)/12, dV/dy = -P(x,z)/12,
+// dV/dz = -P(y,x)/12.
+template <typename T, int A0, int A1, int A2, int A3, int A4, int A5>
+LG_DEV T voluder_form(const T (&a)[8], const T (&b)[8])
+{
+   return (a[A1] + a[A2]) * (b[A0] + b[A1]) - (a[A0] + a[A1]) * (b[A1] + b[A2]) +
+          (a[A0] + a[A4]) * (b[A3] + b[A4]) - (a[A3] + a[A4]) * (b[A0] + b[A4]) -
+          (a[A2] + a[A5]) * (b[A3] + b[A5]) + (a[A3] + a[A5]) * (b[A2] + b[A5]);
+}
+
+template <typename T, int N, int A0, int A1, int A2, int A3, int A4, int A5>
+LG_DEV void voluder_node(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&dx)[8],
+                         T (&dy)[8], T (&dz)[8])
+{
+   const T twelfth = T(1.0) / T(12.0);
+   dx[N] = voluder_form<T, A0, A1, A2, A3, A4, A5>(y, z) * twelfth;
+   dy[N] = -(voluder_form<T, A0, A1, A2, A3, A4, A5>(x, z) * twelfth);
+   dz[N] = -(voluder_form<T, A0, A1, A2, A3, A4, A5>(y, x) * twelfth);
+}
+
+// CalcElemVolumeDerivative, lulesh.cc:623-663 (neighbour permutations of
+// each node as the reference passes them to VoluDer).
+template <typename T>
+LG_DEV void volume_derivative(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&dx)[8],
+                              T (&dy)[8], T (&dz)[8])
+{
+   voluder_node<T, 0, 1, 2, 3, 4, 5, 7>(x, y, z, dx, dy, dz);
+   voluder_node<T, 3, 0, 1, 2, 7, 4, 6>(x, y, z, dx, dy, dz);
+   voluder_node<T, 2, 3, 0, 1, 6, 7, 5>(x, y, z, dx, dy, dz);
+   voluder_node<T, 1, 2, 3, 0, 5, 6, 4>(x, y, z, dx, dy, dz);
+   voluder_node<T, 4, 7, 6, 5, 0, 3, 1>(x, y, z, dx, dy, dz);
+   voluder_node<T, 5, 4, 7, 6, 1, 0, 2>(x, y, z, dx, dy, dz);
+   voluder_node<T, 6, 5, 4, 7, 2, 1, 3>(x, y, z, dx, dy, dz);
+   voluder_node<T, 7, 6, 5, 4, 3, 2, 0>(x, y, z, dx, dy, dz);
+}
+
+// Squared-area form of one face, AreaFace (lulesh.cc:1370-1390).
+template <typename T, int A, int B, int C, int D>
+LG_DEV T face_area2(const T (&x)[8], const T (&y)[8], const T (&z)[8])
+{
+   const T fx = (x[C] - x[A]) - (x[D] - x[B]);
+   const T fy = (y[C] - y[A]) - (y[D] - y[B]);
+   const T fz = (z[C] - z[A]) - (z[D] - z[B]);
+   const T gx = (x[C] - x[A]) + (x[D] - x[B]);
+   const T gy = (y[C] - y[A]) + (y[D] - y[B]);
+   const T gz = (z[C] - z[A]) + (z[D] - z[B]);
+   const T fg = fx * gx + fy * gy + fz * gz;
+   return (fx * fx + fy * fy + fz * fz) * (gx * gx + gy * gy + gz * gz) - fg * fg;
+}
+
+template <typename T>
+LG_DEV T keep_max(T a, T m)
+{
+   return (a < m) ? m : a; // std::max(a, charLength), lulesh.cc:1405
+}
+
+// CalcElemCharacteristicLength, lulesh.cc:1394-1435.
+template <typename T>
+LG_DEV T char_length(const T (&x)[8], const T (&y)[8], const T (&z)[8], T volume)
+{
+   T m = T(0);
+   m = keep_max(face_area2<T, 0, 1, 2, 3>(x, y, z), m);
+   m = keep_max(face_area2<T, 4, 5, 6, 7>(x, y, z), m);
+   m = keep_max(face_area2<T, 0, 1, 5, 4>(x, y, z), m);
+   m = keep_max(face_area2<T, 1, 2, 6, 5>(x, y, z), m);
+   m = keep_max(face_area2<T, 2, 3, 7, 6>(x, y, z), m);
+   m = keep_max(face_area2<T, 3, 0, 4, 7>(x, y, z), m);
+   return T(4.0) * volume / sqrt(m);
+}
+
+// Difference of two four-node sums ((a0+a1)+a2)+a3 - (((b0+b1)+b2)+b3), the
+// face-centred differences of lulesh.cc:1691-1753.
+template <typename T, int A0, int A1, int A2, int A3, int B0, int B1, int B2, int B3>
+LG_DEV T face_diff(const T (&c)[8])
+{
+   return (c[A0] + c[A1] + c[A2] + c[A3]) - (c[B0] + c[B1] + c[B2] + c[B3]);
+}
+
+// Hourglass base vectors, lulesh.cc:743-776: gamma[m][j] = sign bit set
+// where the table holds -1.
+__host__ __device__ constexpr int hg_negative(int m, int j)
+{
+   // bit j of each mask is set where gamma[m][j] = -1
+   return ((m == 0 ? 0x3C : m == 1 ? 0x96 : m == 2 ? 0xAA : 0xA5) >> j) & 1;
+}
+
+template <typename T>
+LG_DEV T hg_gamma(int m, int j)
+{
+   return hg_negative(m, j) ? T(-1.) : T(1.);
+}
+
+} // namespace lgpu

@@ -1,0 +1,92 @@
+// lgpu_kernels.h -- device kernels of the Lagrange leapfrog step.
+//
+// Four passes per cycle, each a single launch over its whole index space:
+//
+//   k_force  (elements)  stress integration + Flanagan-Belytschko hourglass,
+//                        per-corner forces to corner planes
+//   k_node   (nodes)     deterministic corner gather in nodeElemCornerList
+//                        order, acceleration, symmetry BCs, velocity, position
+//   k_kin    (elements)  volume, vnew/delv, characteristic length, strain
+//                        rate, monotonic-Q gradients
+//   k_eos    (region-ordered elements)  qstop check, monotonic-Q limiter,
+//                        EOS with its region repeat count, sound speed,
+//                        volume update, Courant/hydro minima (last-block
+//                        reduction)
+//
+// The Policy template parameter selects the arithmetic of each part:
+// PolicyFp64 keeps everything in FP64 with the reference's association order,
+// PolicyMixed runs force integration, Q gradients/limiter and EOS pressure
+// and sound speed in FP32 on element-local coordinates.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lgpu_geom.h"
+
+namespace lgpu {
+
+// Error sites, in the order the reference checks them within one cycle.
+enum ErrSite {
+   kErrForceDeterm = 0, // lulesh.cc:1082-1091  determ <= 0       -> VolumeError
+   kErrHourglassV = 1,  // lulesh.cc:1033-1040  v <= 0            -> VolumeError
+   kErrKinVnew = 2,     // lulesh.cc:1597-1605  vnew <= 0         -> VolumeError
+   kErrQStop = 3,       // lulesh.cc:1993-2008  q > qstop         -> QStopError
+   kErrMaterialV = 4,   // lulesh.cc:2366-2384  clamped v <= 0    -> VolumeError
+   kNumErrSites = 8
+};
+
+struct Consts {
+   double e_cut, p_cut, q_cut, v_cut, u_cut;
+   double hgcoef, ss4o3, qstop, monoq_max_slope, monoq_limiter_mult;
+   double qlc_monoq, qqc_monoq, qqc, eosvmax, eosvmin, pmin, emin, dvovmax, refdens;
+};
+
+struct PolicyFp64 {
+   using force_t = double;  // force integration arithmetic
+   using corner_t = double; // stored per-corner force
+   using grad_t = double;   // Q gradients + limiter arithmetic and storage
+   using eos_t = double;    // EOS pressure / sound-speed arithmetic
+   static constexpr bool kLocalCoords = false;
+};
+
+struct PolicyMixed {
+   using force_t = float;
+   using corner_t = float;
+   using grad_t = float;
+   using eos_t = float;
+   static constexpr bool kLocalCoords = true;
+};
+
+// Everything a cycle touches on the device.
+struct DevState {
+   int32_t numElem, numNode, numReg;
+   int32_t xcdRemap; // 1: map consecutive blocks onto one XCD's L2
+   // node fields
+   double *x, *y, *z, *xd, *yd, *zd, *nodalMass;
+   double *fx, *fy, *fz, *xdd, *ydd, *zdd; // diagnostics (may be null)
+   const uint8_t *nodeBC;                  // bit0 symmX, bit1 symmY, bit2 symmZ
+   const int32_t *nodeElemStart, *cornerList;
+   // element fields
+   const int32_t *nodelist;
+   const int32_t *lxim, *lxip, *letam, *letap, *lzetam, *lzetap, *elemBC;
+   double *e, *p, *q, *v, *volo, *ss, *elemMass;
+   double *vnew, *delv, *arealg, *vdov;
+   double *ql, *qq; // diagnostics (may be null)
+   void *delx_xi, *delx_eta, *delx_zeta, *delv_xi, *delv_eta, *delv_zeta; // grad_t
+   // corner planes: [(component*2 + kind)*8 + corner][elem], kind 0 stress, 1 hourglass
+   void *corner;
+   // region order
+   const int32_t *perm;     // region-major element order
+   const int32_t *regStart; // numReg+1 offsets into perm
+   const int32_t *regRep;   // EOS repeat count per region
+   // control
+   int32_t *err;            // kNumErrSites flags
+   double *dtPartial;       // 2 per k_eos block
+   uint32_t *dtCounter;     // blocks done
+   double *dtOut;           // [0] dtcourant, [1] dthydro
+   const double *dtDevice;  // device-side deltatime (null: use argument)
+   const int32_t *active;   // device-side loop flag (null: always active)
+};
+
+} // namespace lgpu

@@ -1,0 +1,32 @@
+// lgpu_launch.h -- host-side launchers of the step kernels, one explicit
+// instantiation per precision policy (lgpu_step_fp64.hip, lgpu_step_mixed.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "lgpu_kernels.h"
+
+namespace lgpu {
+
+constexpr int kEosBlock = 256;
+
+inline unsigned eos_blocks(int numElem)
+{
+   return (unsigned)((numElem + kEosBlock - 1) / kEosBlock);
+}
+
+template <class P>
+struct Launch {
+   static void force(const DevState &s, const Consts &k, int64_t planeStride, int block,
+                     hipStream_t st);
+   static void node(const DevState &s, const Consts &k, int64_t planeStride, double dt, int block,
+                    hipStream_t st);
+   static void kin(const DevState &s, const Consts &k, double dt, int block, hipStream_t st);
+   static void eos(const DevState &s, const Consts &k, hipStream_t st);
+};
+
+extern template struct Launch<PolicyFp64>;
+extern template struct Launch<PolicyMixed>;
+
+} // namespace lgpu

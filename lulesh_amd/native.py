@@ -1,0 +1,335 @@
+"""ctypes bindings of the native libraries (include/lulesh_gpu.h, include/lulesh_host.h).
+
+The device path has no Python or CPU fallback: if the built library is
+missing, loading raises.
+"""
+from __future__ import annotations
+
+import ctypes
+from pathlib import Path
+
+import numpy as np
+
+PKG = Path(__file__).resolve().parent
+LIBDIR = PKG / "lib"
+GPU_SO = LIBDIR / "liblulesh_gpu.so"
+HOST_SO = LIBDIR / "liblulesh_host.so"
+DRIVER = LIBDIR / "lulesh2.0-mi355x"
+
+FIELDS = [
+    "x", "y", "z", "xd", "yd", "zd", "xdd", "ydd", "zdd", "fx", "fy", "fz", "nodalMass",
+    "e", "p", "q", "ql", "qq", "v", "volo", "vnew", "delv", "vdov", "arealg", "ss",
+    "elemMass", "delx_xi", "delx_eta", "delx_zeta", "delv_xi", "delv_eta", "delv_zeta",
+]
+NODE_FIELDS = set(FIELDS[:13])
+FP64, MIXED = 0, 1
+OPT_KEEP_DIAGNOSTICS, OPT_KERNEL_TIMING, OPT_BLOCK_SIZE = 1, 2, 3
+STATUS = {0: "ok", -1: "VolumeError", -2: "QStopError", -100: "EINVAL", -101: "EDEVICE",
+          -102: "ENOMEM"}
+
+_i32p = ctypes.POINTER(ctypes.c_int32)
+_f64p = ctypes.POINTER(ctypes.c_double)
+
+
+class Constants(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "e_cut", "p_cut", "q_cut", "v_cut", "u_cut", "hgcoef", "ss4o3", "qstop",
+        "monoq_max_slope", "monoq_limiter_mult", "qlc_monoq", "qqc_monoq", "qqc", "eosvmax",
+        "eosvmin", "pmin", "emin", "dvovmax", "refdens")]
+
+
+class MeshDesc(ctypes.Structure):
+    _fields_ = [
+        ("numElem", ctypes.c_int32), ("numNode", ctypes.c_int32),
+        ("sizeX", ctypes.c_int32), ("sizeY", ctypes.c_int32), ("sizeZ", ctypes.c_int32),
+        ("nodelist", _i32p),
+        ("lxim", _i32p), ("lxip", _i32p), ("letam", _i32p), ("letap", _i32p),
+        ("lzetam", _i32p), ("lzetap", _i32p), ("elemBC", _i32p),
+        ("nodeElemStart", _i32p), ("nodeElemCornerList", _i32p),
+        ("symmX", _i32p), ("symmY", _i32p), ("symmZ", _i32p),
+        ("numSymmX", ctypes.c_int32), ("numSymmY", ctypes.c_int32), ("numSymmZ", ctypes.c_int32),
+        ("numReg", ctypes.c_int32),
+        ("regElemSize", _i32p), ("regElemlist", ctypes.POINTER(_i32p)),
+        ("cost", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("k", Constants),
+    ]
+
+
+class TimeState(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "time", "deltatime", "dtcourant", "dthydro", "stoptime", "dtfixed", "dtmax",
+        "deltatimemultlb", "deltatimemultub")] + [("cycle", ctypes.c_int32)]
+
+
+class Info(ctypes.Structure):
+    _fields_ = [("path", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("numElem", ctypes.c_int32), ("numNode", ctypes.c_int32),
+                ("numReg", ctypes.c_int32), ("block", ctypes.c_int32),
+                ("arena_bytes", ctypes.c_int64)]
+
+
+_GPU: ctypes.CDLL | None = None
+_HOST: ctypes.CDLL | None = None
+
+
+def gpu_lib() -> ctypes.CDLL:
+    """The device library; raises if it has not been built."""
+    global _GPU
+    if _GPU is None:
+        if not GPU_SO.exists():
+            raise RuntimeError(f"{GPU_SO} is missing: build it with `python -m lulesh_amd.build` "
+                               "(there is no CPU fallback)")
+        lib = ctypes.CDLL(str(GPU_SO))
+        vp, i32, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+        lib.lgpu_default_constants.argtypes = [ctypes.POINTER(Constants)]
+        lib.lgpu_default_constants.restype = None
+        lib.lgpu_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(vp)]
+        lib.lgpu_create.restype = ctypes.c_int
+        lib.lgpu_destroy.argtypes = [vp]
+        lib.lgpu_destroy.restype = None
+        for n in ("lgpu_upload", "lgpu_download"):
+            getattr(lib, n).argtypes = [vp, ctypes.c_int, _f64p]
+            getattr(lib, n).restype = ctypes.c_int
+        for n in ("lgpu_lagrange_nodal", "lgpu_lagrange_elements"):
+            getattr(lib, n).argtypes = [vp, dbl]
+            getattr(lib, n).restype = ctypes.c_int
+        lib.lgpu_time_constraints.argtypes = [vp, _f64p, _f64p]
+        lib.lgpu_time_constraints.restype = ctypes.c_int
+        lib.lgpu_run.argtypes = [vp, ctypes.POINTER(TimeState), i32]
+        lib.lgpu_run.restype = ctypes.c_int
+        lib.lgpu_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int64]
+        lib.lgpu_set_option.restype = ctypes.c_int
+        lib.lgpu_num_kernels.argtypes = []
+        lib.lgpu_num_kernels.restype = ctypes.c_int
+        lib.lgpu_kernel_name.argtypes = [ctypes.c_int]
+        lib.lgpu_kernel_name.restype = ctypes.c_char_p
+        lib.lgpu_kernel_stats.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64), _f64p]
+        lib.lgpu_kernel_stats.restype = ctypes.c_int
+        lib.lgpu_reset_kernel_stats.argtypes = [vp]
+        lib.lgpu_reset_kernel_stats.restype = ctypes.c_int
+        lib.lgpu_get_info.argtypes = [vp, ctypes.POINTER(Info)]
+        lib.lgpu_get_info.restype = ctypes.c_int
+        lib.lgpu_last_error.argtypes = [vp]
+        lib.lgpu_last_error.restype = ctypes.c_char_p
+        _GPU = lib
+    return _GPU
+
+
+def host_lib() -> ctypes.CDLL:
+    global _HOST
+    if _HOST is None:
+        gpu_lib()
+        if not HOST_SO.exists():
+            raise RuntimeError(f"{HOST_SO} is missing: build it with `python -m lulesh_amd.build`")
+        lib = ctypes.CDLL(str(HOST_SO))
+        vp, i32, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_double
+        lib.lh_create.argtypes = [i32, i32, i32, i32, dbl]
+        lib.lh_create.restype = vp
+        lib.lh_destroy.argtypes = [vp]
+        lib.lh_destroy.restype = None
+        lib.lh_attach_device.argtypes = [vp, i32]
+        lib.lh_attach_device.restype = ctypes.c_int
+        lib.lh_device.argtypes = [vp]
+        lib.lh_device.restype = vp
+        lib.lh_step.argtypes = [vp]
+        lib.lh_step.restype = ctypes.c_int
+        lib.lh_run.argtypes = [vp, i32, i32]
+        lib.lh_run.restype = ctypes.c_int
+        lib.lh_sync_from_device.argtypes = [vp]
+        lib.lh_sync_from_device.restype = ctypes.c_int
+        for n in ("lh_num_elem", "lh_num_node", "lh_num_reg"):
+            getattr(lib, n).argtypes = [vp]
+            getattr(lib, n).restype = i32
+        lib.lh_real.argtypes = [vp, i32]
+        lib.lh_real.restype = _f64p
+        lib.lh_int.argtypes = [vp, i32, i32, ctypes.POINTER(ctypes.c_int64)]
+        lib.lh_int.restype = _i32p
+        lib.lh_get.argtypes = [vp, i32]
+        lib.lh_get.restype = dbl
+        lib.lh_set.argtypes = [vp, i32, dbl]
+        lib.lh_set.restype = None
+        lib.lh_symmetry.argtypes = [vp, _f64p]
+        lib.lh_symmetry.restype = None
+        _HOST = lib
+    return _HOST
+
+
+def check(status: int, ctx=None) -> int:
+    """Raises on usage/device errors; returns physics status codes (0, -1, -2)."""
+    if status <= -100:
+        msg = ""
+        if ctx:
+            msg = gpu_lib().lgpu_last_error(ctx).decode()
+        raise RuntimeError(f"lulesh_gpu error {STATUS.get(status, status)} {msg}")
+    return status
+
+
+def default_constants() -> Constants:
+    k = Constants()
+    gpu_lib().lgpu_default_constants(ctypes.byref(k))
+    return k
+
+
+def _i32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+class DeviceContext:
+    """An lgpu_ctx built from numpy arrays (mesh description of lulesh_gpu.h)."""
+
+    def __init__(self, *, nodelist, lxim, lxip, letam, letap, lzetam, lzetap, elemBC,
+                 nodeElemStart, nodeElemCornerList, symmX, symmY, symmZ, regions, cost,
+                 size, precision=FP64, constants: Constants | None = None):
+        lib = gpu_lib()
+        self._keep = [_i32(a) for a in (nodelist, lxim, lxip, letam, letap, lzetam, lzetap,
+                                        elemBC, nodeElemStart, nodeElemCornerList, symmX,
+                                        symmY, symmZ)]
+        (nl, a1, a2, a3, a4, a5, a6, bc, st, cl, sx, sy, sz) = self._keep
+        self._regions = [_i32(r) for r in regions]
+        self._regsize = _i32([len(r) for r in self._regions])
+        self._regptrs = (_i32p * len(self._regions))(
+            *[r.ctypes.data_as(_i32p) for r in self._regions])
+        d = MeshDesc()
+        d.numElem = len(elemBC)
+        d.numNode = len(nodeElemStart) - 1
+        d.sizeX = d.sizeY = d.sizeZ = size
+        for name, arr in (("nodelist", nl), ("lxim", a1), ("lxip", a2), ("letam", a3),
+                          ("letap", a4), ("lzetam", a5), ("lzetap", a6), ("elemBC", bc),
+                          ("nodeElemStart", st), ("nodeElemCornerList", cl), ("symmX", sx),
+                          ("symmY", sy), ("symmZ", sz)):
+            setattr(d, name, arr.ctypes.data_as(_i32p))
+        d.numSymmX, d.numSymmY, d.numSymmZ = len(sx), len(sy), len(sz)
+        d.numReg = len(self._regions)
+        d.regElemSize = self._regsize.ctypes.data_as(_i32p)
+        d.regElemlist = self._regptrs
+        d.cost = cost
+        d.precision = precision
+        d.k = constants if constants is not None else default_constants()
+        self.numElem, self.numNode = d.numElem, d.numNode
+        self.precision = precision
+        ptr = ctypes.c_void_p()
+        check(lib.lgpu_create(ctypes.byref(d), ctypes.byref(ptr)))
+        self.ptr = ptr
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            gpu_lib().lgpu_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def _n(self, name: str) -> int:
+        return self.numNode if name in NODE_FIELDS else self.numElem
+
+    def upload(self, name: str, values) -> None:
+        a = np.ascontiguousarray(values, dtype=np.float64)
+        assert a.shape == (self._n(name),), (name, a.shape)
+        check(gpu_lib().lgpu_upload(self.ptr, FIELDS.index(name), a.ctypes.data_as(_f64p)), self.ptr)
+
+    def download(self, name: str) -> np.ndarray:
+        out = np.empty(self._n(name), np.float64)
+        check(gpu_lib().lgpu_download(self.ptr, FIELDS.index(name), out.ctypes.data_as(_f64p)), self.ptr)
+        return out
+
+    def set_option(self, option: int, value: int) -> None:
+        check(gpu_lib().lgpu_set_option(self.ptr, option, value), self.ptr)
+
+    def lagrange_nodal(self, dt: float) -> int:
+        return check(gpu_lib().lgpu_lagrange_nodal(self.ptr, dt), self.ptr)
+
+    def lagrange_elements(self, dt: float) -> int:
+        return check(gpu_lib().lgpu_lagrange_elements(self.ptr, dt), self.ptr)
+
+    def time_constraints(self) -> tuple[int, float, float]:
+        c, h = ctypes.c_double(), ctypes.c_double()
+        st = check(gpu_lib().lgpu_time_constraints(self.ptr, ctypes.byref(c), ctypes.byref(h)),
+                   self.ptr)
+        return st, c.value, h.value
+
+    def run(self, ts: TimeState, max_cycle: int) -> int:
+        return check(gpu_lib().lgpu_run(self.ptr, ctypes.byref(ts), max_cycle), self.ptr)
+
+    def kernel_stats(self) -> dict[str, tuple[int, float]]:
+        lib = gpu_lib()
+        out = {}
+        for k in range(lib.lgpu_num_kernels()):
+            n, ms = ctypes.c_int64(), ctypes.c_double()
+            check(lib.lgpu_kernel_stats(self.ptr, k, ctypes.byref(n), ctypes.byref(ms)), self.ptr)
+            out[lib.lgpu_kernel_name(k).decode()] = (n.value, ms.value)
+        return out
+
+    def reset_kernel_stats(self) -> None:
+        check(gpu_lib().lgpu_reset_kernel_stats(self.ptr), self.ptr)
+
+    def info(self) -> Info:
+        i = Info()
+        check(gpu_lib().lgpu_get_info(self.ptr, ctypes.byref(i)), self.ptr)
+        return i
+
+
+INT_ARRAYS = ["nodelist", "lxim", "lxip", "letam", "letap", "lzetam", "lzetap", "elemBC",
+              "nodeElemStart", "nodeElemCornerList", "symmX", "symmY", "symmZ", "regNumList",
+              "regElemSize", "regElemlist"]
+SCALARS = ["time", "deltatime", "dtcourant", "dthydro", "cycle", "stoptime", "dtfixed", "dtmax",
+           "deltatimemultlb", "deltatimemultub"]
+
+
+class HostDomain:
+    """The C++ host Domain (lulesh_amd/csrc/host/domain.h) through its C API."""
+
+    def __init__(self, s: int, nr: int = 11, balance: int = 1, cost: int = 1,
+                 emult: float = 1.0):
+        self._lib = host_lib()
+        self.ptr = self._lib.lh_create(s, nr, balance, cost, float(emult))
+        if not self.ptr:
+            raise ValueError("lh_create failed")
+        self.s = s
+        self.numElem = self._lib.lh_num_elem(self.ptr)
+        self.numNode = self._lib.lh_num_node(self.ptr)
+        self.numReg = self._lib.lh_num_reg(self.ptr)
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            self._lib.lh_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def real(self, name: str) -> np.ndarray:
+        n = self.numNode if name in NODE_FIELDS else self.numElem
+        return np.ctypeslib.as_array(self._lib.lh_real(self.ptr, FIELDS.index(name)), shape=(n,))
+
+    def ints(self, name: str, region: int = 0) -> np.ndarray:
+        ln = ctypes.c_int64()
+        p = self._lib.lh_int(self.ptr, INT_ARRAYS.index(name), region, ctypes.byref(ln))
+        if not p or ln.value == 0:
+            return np.zeros(0, np.int32)
+        return np.ctypeslib.as_array(p, shape=(ln.value,))
+
+    def get(self, name: str) -> float:
+        return self._lib.lh_get(self.ptr, SCALARS.index(name))
+
+    def set(self, name: str, v: float) -> None:
+        self._lib.lh_set(self.ptr, SCALARS.index(name), float(v))
+
+    def attach_device(self, precision: int = FP64) -> None:
+        check(self._lib.lh_attach_device(self.ptr, precision))
+
+    def device(self):
+        return self._lib.lh_device(self.ptr)
+
+    def step(self) -> int:
+        return check(self._lib.lh_step(self.ptr))
+
+    def run(self, its: int, device_loop: bool = False) -> int:
+        return check(self._lib.lh_run(self.ptr, its, int(bool(device_loop))))
+
+    def sync(self) -> None:
+        check(self._lib.lh_sync_from_device(self.ptr))
+
+    def symmetry(self) -> tuple[float, float, float]:
+        out = (ctypes.c_double * 3)()
+        self._lib.lh_symmetry(self.ptr, out)
+        return out[0], out[1], out[2]

@@ -1,0 +1,56 @@
+"""The C-ABI libraries load (no GPU needed) and export every entry point
+their headers declare."""
+import ctypes
+import re
+from pathlib import Path
+
+import pytest
+
+import lulesh_amd as la
+from lulesh_amd import native
+
+ROOT = Path(__file__).resolve().parent.parent
+
+
+def declared(header: str) -> list[str]:
+    text = (ROOT / "include" / header).read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    names = re.findall(r"\b(l(?:gpu|h)_[a-z_]+)\s*\(", text)
+    return sorted(set(names))
+
+
+@pytest.mark.parametrize("header,loader", [("lulesh_gpu.h", native.gpu_lib),
+                                           ("lulesh_host.h", native.host_lib)])
+def test_exports(header, loader):
+    lib = loader()
+    names = declared(header)
+    assert len(names) >= 10
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def test_field_numbering_matches_oracle():
+    from oracle_binding import REAL_FIELDS
+    assert la.FIELDS == REAL_FIELDS
+
+
+def test_default_constants_are_the_references():
+    k = la.default_constants()
+    # lulesh-init.cc:20-38
+    assert (k.e_cut, k.p_cut, k.q_cut, k.v_cut, k.u_cut) == (1e-7, 1e-7, 1e-7, 1e-10, 1e-7)
+    assert (k.hgcoef, k.qstop, k.monoq_max_slope, k.monoq_limiter_mult) == (3.0, 1e12, 1.0, 2.0)
+    assert (k.qlc_monoq, k.qqc_monoq, k.qqc) == (0.5, 2.0 / 3.0, 2.0)
+    assert (k.eosvmax, k.eosvmin, k.pmin, k.emin, k.dvovmax, k.refdens) == (
+        1e9, 1e-9, 0.0, -1e15, 0.1, 1.0)
+    assert k.ss4o3 == 4.0 / 3.0
+
+
+def test_create_rejects_bad_mesh_without_touching_the_device():
+    """Validation happens on the host before any allocation: an
+    out-of-range node index is refused with EINVAL."""
+    from oracle_binding import OracleDomain
+    from helpers import context_from_oracle
+    o = OracleDomain(3)
+    o.ints("nodelist")[5] = 10_000
+    with pytest.raises(RuntimeError, match="EINVAL"):
+        context_from_oracle(o)
