@@ -22,9 +22,9 @@ using namespace lgpu;
 
 namespace {
 
-enum KernelId { kKForce = 0, kKNode, kKKin, kKEos, kKTimeIncrement, kNumKernels };
+enum KernelId { kKForce = 0, kKNode, kKKin, kKEos, kKDtReduce, kKTimeIncrement, kNumKernels };
 const char *const kKernelNames[kNumKernels] = {"k_force", "k_node", "k_kin", "k_eos",
-                                               "k_time_increment"};
+                                               "k_dt_reduce", "k_time_increment"};
 
 constexpr size_t kAlign = 256;
 
@@ -264,7 +264,9 @@ int launch_elements(lgpu_ctx *c, double dt)
 {
    int r = timed(c, kKKin, [&] { Launch<P>::kin(c->s, c->k, dt, c->block, c->stream); });
    if (r) return r;
-   return timed(c, kKEos, [&] { Launch<P>::eos(c->s, c->k, c->stream); });
+   r = timed(c, kKEos, [&] { Launch<P>::eos(c->s, c->k, c->stream); });
+   if (r) return r;
+   return timed(c, kKDtReduce, [&] { Launch<P>::dt_reduce(c->s, c->stream); });
 }
 
 int resolve_status(const int32_t *err)
@@ -414,7 +416,6 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    add((void **)&s.regRep, sizeof(int32_t) * (size_t)d->numReg);
    add((void **)&s.err, sizeof(int32_t) * kNumErrSites);
    add((void **)&s.dtPartial, sizeof(double) * 2 * eosBlocks);
-   add((void **)&s.dtCounter, sizeof(uint32_t));
    add((void **)&s.dtOut, sizeof(double) * 2);
    add((void **)&c->devTime, sizeof(DevTime));
    size_t total = 0;

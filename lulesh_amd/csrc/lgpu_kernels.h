@@ -10,8 +10,8 @@
 //                        rate, monotonic-Q gradients
 //   k_eos    (region-ordered elements)  qstop check, monotonic-Q limiter,
 //                        EOS with its region repeat count, sound speed,
-//                        volume update, Courant/hydro minima (last-block
-//                        reduction)
+//                        volume update, Courant/hydro block minima
+//   k_dt_reduce (one block) Courant/hydro minima over the block partials
 //
 // The Policy template parameter selects the arithmetic of each part:
 // PolicyFp64 keeps everything in FP64 with the reference's association order,
@@ -83,7 +83,6 @@ struct DevState {
    // control
    int32_t *err;            // kNumErrSites flags
    double *dtPartial;       // 2 per k_eos block
-   uint32_t *dtCounter;     // blocks done
    double *dtOut;           // [0] dtcourant, [1] dthydro
    const double *dtDevice;  // device-side deltatime (null: use argument)
    const int32_t *active;   // device-side loop flag (null: always active)

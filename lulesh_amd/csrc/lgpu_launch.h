@@ -24,6 +24,7 @@ struct Launch {
                     hipStream_t st);
    static void kin(const DevState &s, const Consts &k, double dt, int block, hipStream_t st);
    static void eos(const DevState &s, const Consts &k, hipStream_t st);
+   static void dt_reduce(const DevState &s, hipStream_t st);
 };
 
 extern template struct Launch<PolicyFp64>;
