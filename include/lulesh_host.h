@@ -58,6 +58,10 @@ void lh_set(lh_domain *d, int32_t which, double value);
  * (lulesh-util.cc:197-212) on the host copy of e. */
 void lh_symmetry(lh_domain *d, double out[3]);
 
+/* The first n values of glibc's rand() after srand(seed), from the host's
+ * own generator (the source of the region layout, lulesh-init.cc:406-480). */
+void lh_glibc_rand(uint32_t seed, int32_t n, int32_t *out);
+
 #ifdef __cplusplus
 }
 #endif

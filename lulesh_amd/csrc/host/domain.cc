@@ -571,4 +571,10 @@ void lh_symmetry(lh_domain *h, double out[3])
    out[2] = maxRel;
 }
 
+void lh_glibc_rand(uint32_t seed, int32_t n, int32_t *out)
+{
+   lhost::GlibcRand g(seed);
+   for (int32_t i = 0; i < n; ++i) out[i] = g.next();
+}
+
 } // extern "C"

@@ -150,6 +150,8 @@ def host_lib() -> ctypes.CDLL:
         lib.lh_set.restype = None
         lib.lh_symmetry.argtypes = [vp, _f64p]
         lib.lh_symmetry.restype = None
+        lib.lh_glibc_rand.argtypes = [ctypes.c_uint32, i32, _i32p]
+        lib.lh_glibc_rand.restype = None
         _HOST = lib
     return _HOST
 

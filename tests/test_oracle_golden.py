@@ -1,0 +1,99 @@
+"""Pins the oracle to the reference's own outputs (CPU only).
+
+* ref_phases_*.npz: the full state after every phase of the first cycles,
+  written by the reference build (tests/golden/make_golden.py).  The oracle
+  must reproduce every field bit for bit.
+* ref_summaries.json: the reference's end-of-run numbers at 17 digits.  The
+  oracle must reproduce iteration counts, final origin energy and the
+  symmetry figures exactly.
+* The single-threaded reference build sums forces in element order; its s=30
+  result (2.02507478507199383e+05, recorded from that build in BASELINE.md's
+  measurements) pins the oracle's scatter order.
+"""
+import numpy as np
+import pytest
+
+from helpers import GOLDEN, summaries
+from oracle_binding import NODE_FIELDS, REAL_FIELDS, OracleDomain
+
+STATE = REAL_FIELDS[:26]
+GRADS = REAL_FIELDS[26:]
+SETUP_INTS = ["nodelist", "lxim", "lxip", "letam", "letap", "lzetam", "lzetap", "elemBC",
+              "regNumList", "nodeElemStart", "nodeElemCornerList", "symmX", "symmY", "symmZ",
+              "regElemSize"]
+
+
+def load(name):
+    d = np.load(GOLDEN / name)
+    return {k.replace("__", "/"): d[k] for k in d.files}
+
+
+@pytest.mark.parametrize("name,s,r,b,c,cycles", [
+    ("ref_phases_s4.npz", 4, 11, 1, 1, [1, 2, 3]),
+    ("ref_phases_s6_r21.npz", 6, 21, 2, 2, [1, 2]),
+])
+def test_phase_dumps_bit_exact(name, s, r, b, c, cycles):
+    ref = load(name)
+    o = OracleDomain(s, r, b, c)
+    for k in SETUP_INTS:
+        assert np.array_equal(o.ints(k), ref["setup/" + k]), k
+    for rr in range(o.numReg):
+        assert np.array_equal(o.ints("regElemlist", rr), ref[f"setup/regElemlist{rr}"])
+    for f in STATE:
+        assert np.array_equal(o.real(f), ref["init/" + f]), f
+    assert o.get("deltatime") == ref["init/deltatime"][0]
+    for cyc in cycles:
+        o.call("time_increment")
+        assert o.get("deltatime") == ref[f"c{cyc}/deltatime_used"][0]
+        assert o.call("lagrange_nodal") == 0
+        for f in STATE:
+            assert np.array_equal(o.real(f), ref[f"c{cyc}/nodal/{f}"]), (cyc, "nodal", f)
+        assert o.call("calc_lagrange_elements") == 0
+        assert o.call("calc_q") == 0
+        for f in GRADS:
+            assert np.array_equal(o.real(f), ref[f"c{cyc}/grad/{f}"]), (cyc, "grad", f)
+        assert o.call("apply_material_properties") == 0
+        o.call("update_volumes")
+        for f in STATE:
+            assert np.array_equal(o.real(f), ref[f"c{cyc}/elements/{f}"]), (cyc, "elements", f)
+        o.call("calc_time_constraints")
+        assert o.get("dtcourant") == ref[f"c{cyc}/dt/dtcourant"][0]
+        assert o.get("dthydro") == ref[f"c{cyc}/dt/dthydro"][0]
+
+
+def _cheap(v):
+    its = v["its"] if v["its"] < 9999999 else v["cycles"]
+    return v["s"] ** 3 * its <= 3_000_000
+
+
+CHEAP = sorted(k for k, v in summaries().items() if _cheap(v))
+
+
+@pytest.mark.parametrize("key", CHEAP)
+def test_end_of_run_numbers_exact(key):
+    v = summaries()[key]
+    o = OracleDomain(v["s"], v["r"], v["b"], v["c"], v["emult"])
+    assert o.run(v["its"]) == 0
+    assert int(o.get("cycle")) == v["cycles"]
+    assert f"{o.real('e')[0]:.17e}" == v["e0"]
+    sym = o.symmetry()
+    assert f"{sym[0]:.17e}" == v["maxabs"]
+    assert f"{sym[1]:.17e}" == v["totabs"]
+    assert f"{sym[2]:.17e}" == v["maxrel"]
+    assert f"{o.get('time'):.17e}" == v["time"]
+
+
+def test_s30_to_stoptime_exact():
+    v = summaries()["s30_r11_b1_c1_i9999999_m1"]
+    o = OracleDomain(30)
+    assert o.run(9999999) == 0
+    assert int(o.get("cycle")) == v["cycles"] == 932
+    assert f"{o.real('e')[0]:.17e}" == v["e0"] == "2.02507478507199645e+05"
+
+
+def test_scatter_order_matches_single_thread_reference():
+    o = OracleDomain(30)
+    o.set_scatter(True)
+    assert o.run(9999999) == 0
+    assert int(o.get("cycle")) == 932
+    assert f"{o.real('e')[0]:.17e}" == "2.02507478507199383e+05"
