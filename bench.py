@@ -44,6 +44,9 @@ def kernel_bytes(name: str, s: int, precision: str) -> float:
         "k_force": 32 + 48 * r + 48 + 48 * cb,
         # start 4r, corner list 32, 48 corner values, mass 8r, bc 1r, x..zd r/w 96r
         "k_node": 4 * r + 32 + 48 * cb + 8 * r + 1 * r + 96 * r,
+        # fused brick nodal pass: x..zd read 48r, p q v volo ss elemMass 48,
+        # nodalMass 8r, node BC mask 1r, x..zd written 48r
+        "k_nodal_brick": 48 * r + 48 + 8 * r + 1 * r + 48 * r,
         # nodelist 32, x..zd 48r, volo v 16 -> vnew delv arealg vdov 32, grads 6
         "k_kin": 32 + 48 * r + 16 + 32 + 6 * gb,
         # perm 4, elemBC 4, 6 nbr 24, grads 6, 10 fields 80 -> 5 fields 40

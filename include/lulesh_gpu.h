@@ -84,8 +84,17 @@ typedef struct lgpu_mesh_desc {
    const int32_t *const *regElemlist;         /* numReg lists                */
    int32_t cost;                              /* EOS repeat rule, lulesh.cc:2390-2400 */
    int32_t precision;                         /* enum lgpu_precision         */
+   int32_t flags;                             /* LGPU_FLAG_* below           */
    lgpu_constants k;
 } lgpu_mesh_desc;
+
+/* lgpu_mesh_desc.flags */
+enum lgpu_flags {
+   /* Use the general CSR-gather kernels even when the mesh is the
+    * reference's structured brick (lulesh-init.cc:217-337), for which the
+    * fused brick kernel is chosen by default. */
+   LGPU_FLAG_GENERIC = 1
+};
 
 /* Fills *k with the reference's constants (lulesh-init.cc:20-38). */
 void lgpu_default_constants(lgpu_constants *k);
@@ -144,8 +153,13 @@ int lgpu_run(lgpu_ctx *ctx, lgpu_time_state *ts, int32_t max_cycle);
 enum lgpu_option {
    LGPU_OPT_KEEP_DIAGNOSTICS = 1, /* also store fx,fy,fz,xdd,ydd,zdd,ql,qq  */
    LGPU_OPT_KERNEL_TIMING = 2,    /* time every kernel with HIP events       */
-   LGPU_OPT_BLOCK_SIZE = 3        /* threads per block of the element/node
-                                     kernels: 64, 128, 256, 512 or 1024      */
+   LGPU_OPT_BLOCK_SIZE = 3,       /* threads per block of the element/node
+                                     kernels of the generic path: 64, 128
+                                     or 256                                  */
+   LGPU_OPT_BRICK_CHUNK = 4,      /* node planes per workgroup of the fused
+                                     brick kernel (1..mesh edge)             */
+   LGPU_OPT_XCD_REMAP = 5         /* 1 (default): contiguous block ranges per
+                                     XCD; 0: hardware order                 */
 };
 int lgpu_set_option(lgpu_ctx *ctx, int option, int64_t value);
 

@@ -22,9 +22,12 @@ using namespace lgpu;
 
 namespace {
 
-enum KernelId { kKForce = 0, kKNode, kKKin, kKEos, kKDtReduce, kKTimeIncrement, kNumKernels };
+enum KernelId {
+   kKForce = 0, kKNode, kKKin, kKEos, kKDtReduce, kKTimeIncrement, kKNodalBrick, kNumKernels
+};
 const char *const kKernelNames[kNumKernels] = {"k_force", "k_node", "k_kin", "k_eos",
-                                               "k_dt_reduce", "k_time_increment"};
+                                               "k_dt_reduce", "k_time_increment",
+                                               "k_nodal_brick"};
 
 constexpr size_t kAlign = 256;
 
@@ -185,6 +188,58 @@ int validate(const lgpu_mesh_desc *d, std::string &why)
    return LGPU_OK;
 }
 
+FastDiv make_fastdiv(uint32_t d)
+{
+   FastDiv f;
+   f.d = d;
+   f.m = UINT64_MAX / d + 1;
+   return f;
+}
+
+// True when the description is the reference's structured brick: lattice
+// element-to-node map (lulesh-init.cc:245-266) and node-to-corner lists in
+// ascending corner order (lulesh-init.cc:271-319).
+bool is_reference_brick(const lgpu_mesh_desc *d)
+{
+   const int64_t s = d->sizeX;
+   if (s < 1 || d->sizeY != s || d->sizeZ != s) return false;
+   const int64_t en = s + 1;
+   if (d->numElem != s * s * s || d->numNode != en * en * en) return false;
+   int64_t e = 0;
+   for (int64_t c = 0; c < s; ++c)
+      for (int64_t b = 0; b < s; ++b)
+         for (int64_t a = 0; a < s; ++a, ++e) {
+            const int64_t n0 = (c * en + b) * en + a;
+            const int64_t want[8] = {n0, n0 + 1, n0 + en + 1, n0 + en, n0 + en * en,
+                                     n0 + en * en + 1, n0 + en * en + en + 1, n0 + en * en + en};
+            for (int j = 0; j < 8; ++j)
+               if (d->nodelist[8 * e + j] != want[j]) return false;
+         }
+   // Corner lists: every node's corners in ascending order.
+   const int64_t N = d->numNode;
+   std::vector<int32_t> cnt((size_t)N, 0);
+   for (int64_t c = 0; c < 8 * (int64_t)d->numElem; ++c) cnt[(size_t)d->nodelist[c]]++;
+   int64_t start = 0;
+   for (int64_t n = 0; n < N; ++n) {
+      if (d->nodeElemStart[n] != start) return false;
+      start += cnt[(size_t)n];
+   }
+   if (d->nodeElemStart[N] != start) return false;
+   std::fill(cnt.begin(), cnt.end(), 0);
+   for (int64_t c = 0; c < 8 * (int64_t)d->numElem; ++c) {
+      const int32_t n = d->nodelist[c];
+      if (d->nodeElemCornerList[d->nodeElemStart[n] + cnt[(size_t)n]++] != (int32_t)c) return false;
+   }
+   return true;
+}
+
+void set_brick_chunk(DevState &s, int chunk)
+{
+   Brick &g = s.brick;
+   g.chunkZ = std::max(1, std::min(chunk, g.en));
+   g.chunks = (g.en + g.chunkZ - 1) / g.chunkZ;
+}
+
 // EOS repeat count of region r, lulesh.cc:2390-2400.
 int32_t region_rep(int32_t r, int32_t numReg, int32_t cost)
 {
@@ -254,6 +309,19 @@ int enable_diagnostics(lgpu_ctx *c)
 template <class P>
 int launch_nodal(lgpu_ctx *c, double dt)
 {
+   if (c->s.isBrick) {
+      const int r = timed(c, kKNodalBrick, [&] { Launch<P>::nodal_brick(c->s, c->k, dt, c->stream); });
+      if (r) return r;
+      // The new node state is in the second buffer: make it current.
+      DevState &s = c->s;
+      std::swap(s.x, s.xo);
+      std::swap(s.y, s.yo);
+      std::swap(s.z, s.zo);
+      std::swap(s.xd, s.xdo);
+      std::swap(s.yd, s.ydo);
+      std::swap(s.zd, s.zdo);
+      return LGPU_OK;
+   }
    int r = timed(c, kKForce, [&] { Launch<P>::force(c->s, c->k, c->planeStride, c->block, c->stream); });
    if (r) return r;
    return timed(c, kKNode, [&] { Launch<P>::node(c->s, c->k, c->planeStride, dt, c->block, c->stream); });
@@ -384,6 +452,7 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
       return fail(LGPU_EDEVICE);
 
+   const bool brick = !(d->flags & LGPU_FLAG_GENERIC) && is_reference_brick(d);
    const size_t E = (size_t)d->numElem, N = (size_t)d->numNode;
    const size_t gradBytes = (d->precision == LGPU_MIXED) ? 4 : 8;
    const size_t cornerBytes = (d->precision == LGPU_MIXED) ? 4 : 8;
@@ -399,6 +468,10 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    auto add = [&](void **dst, size_t bytes) { slots.push_back({dst, align_up(bytes)}); };
    double **nodeReal[] = {&s.x, &s.y, &s.z, &s.xd, &s.yd, &s.zd, &s.nodalMass};
    for (double **p : nodeReal) add((void **)p, sizeof(double) * N);
+   if (brick) {
+      double **second[] = {&s.xo, &s.yo, &s.zo, &s.xdo, &s.ydo, &s.zdo};
+      for (double **p : second) add((void **)p, sizeof(double) * N);
+   }
    add((void **)&s.nodeBC, N);
    add((void **)&s.nodeElemStart, sizeof(int32_t) * (N + 1));
    add((void **)&s.cornerList, sizeof(int32_t) * (size_t)std::max<int64_t>(nCorner, 1));
@@ -410,7 +483,7 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    for (double **p : elemReal) add((void **)p, sizeof(double) * E);
    void **grads[] = {&s.delx_xi, &s.delx_eta, &s.delx_zeta, &s.delv_xi, &s.delv_eta, &s.delv_zeta};
    for (void **p : grads) add(p, gradBytes * E);
-   add(&s.corner, cornerBytes * (size_t)plane * 48);
+   if (!brick) add(&s.corner, cornerBytes * (size_t)plane * 48);
    add((void **)&s.perm, sizeof(int32_t) * E);
    add((void **)&s.regStart, sizeof(int32_t) * (size_t)(d->numReg + 1));
    add((void **)&s.regRep, sizeof(int32_t) * (size_t)d->numReg);
@@ -436,6 +509,16 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    s.numNode = d->numNode;
    s.numReg = d->numReg;
    s.xcdRemap = 1;
+   s.isBrick = brick ? 1 : 0;
+   if (brick) {
+      Brick &g = s.brick;
+      g.s = d->sizeX;
+      g.en = d->sizeX + 1;
+      g.tilesX = g.tilesY = (g.en + 14) / 15;
+      g.divS = make_fastdiv((uint32_t)g.s);
+      g.divS2 = make_fastdiv((uint32_t)(g.s * g.s));
+      set_brick_chunk(s, 8);
+   }
 
    // Host-built tables.
    std::vector<uint8_t> bc(N, 0);
@@ -599,7 +682,6 @@ int lgpu_run(lgpu_ctx *c, lgpu_time_state *ts, int32_t maxCycle)
                                    hipMemcpyHostToDevice, c->stream));
    HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
 
-   DevState saved = c->s;
    c->s.dtDevice = &c->devTime->deltatime;
    c->s.active = &c->devTime->active;
    const int batch = 16;
@@ -622,7 +704,9 @@ int lgpu_run(lgpu_ctx *c, lgpu_time_state *ts, int32_t maxCycle)
       if (c->timing) harvest_timing(c);
       if (!h->active) break;
    }
-   c->s = saved;
+   // Back to host-driven dt (the node buffers keep whatever swaps happened).
+   c->s.dtDevice = nullptr;
+   c->s.active = nullptr;
    if (st) return st;
    hipLaunchKernelGGL(k_time_final, dim3(1), dim3(64), 0, c->stream, c->devTime, c->s.dtOut);
    HIP_OR_RETURN(c, hipGetLastError());
@@ -662,6 +746,13 @@ int lgpu_set_option(lgpu_ctx *c, int option, int64_t value)
       if (value != 64 && value != 128 && value != 256) return LGPU_EINVAL;
       c->block = (int)value;
       return LGPU_OK;
+   case LGPU_OPT_BRICK_CHUNK:
+      if (!c->s.isBrick || value < 1) return LGPU_EINVAL;
+      set_brick_chunk(c->s, (int)std::min<int64_t>(value, 1 << 20));
+      return LGPU_OK;
+   case LGPU_OPT_XCD_REMAP:
+      c->s.xcdRemap = value ? 1 : 0;
+      return LGPU_OK;
    default:
       return LGPU_EINVAL;
    }
@@ -699,7 +790,7 @@ int lgpu_reset_kernel_stats(lgpu_ctx *c)
 int lgpu_get_info(lgpu_ctx *c, lgpu_info *info)
 {
    if (!c || !info) return LGPU_EINVAL;
-   info->path = 0;
+   info->path = c->s.isBrick ? 1 : 0;
    info->precision = c->precision;
    info->numElem = c->numElem;
    info->numNode = c->numNode;

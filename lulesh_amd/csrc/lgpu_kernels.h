@@ -58,12 +58,30 @@ struct PolicyMixed {
    static constexpr bool kLocalCoords = true;
 };
 
+// Division by a run-time constant d (< 2^32) as a 64-bit multiply-high:
+// q = hi64(n * m) with m = floor((2^64 - 1) / d) + 1 is exact for n < 2^32.
+struct FastDiv {
+   uint64_t m;
+   uint32_t d;
+};
+
+// Structured-brick description: s elements per edge, lattice node/element
+// numbering of lulesh-init.cc:217-267.  Set when the mesh is the reference's
+// brick, which lets the nodal pass run as one fused kernel.
+struct Brick {
+   int32_t s, en;          // elements and nodes per edge
+   int32_t tilesX, tilesY; // 15x15-node column tiles
+   int32_t chunkZ, chunks; // node planes per z-chunk, number of chunks
+   FastDiv divS, divS2;    // element index -> (a, b, c)
+};
+
 // Everything a cycle touches on the device.
 struct DevState {
    int32_t numElem, numNode, numReg;
    int32_t xcdRemap; // 1: map consecutive blocks onto one XCD's L2
-   // node fields
+   // node fields (current buffer; the brick path writes the other one)
    double *x, *y, *z, *xd, *yd, *zd, *nodalMass;
+   double *xo, *yo, *zo, *xdo, *ydo, *zdo; // second node buffer (brick path)
    double *fx, *fy, *fz, *xdd, *ydd, *zdd; // diagnostics (may be null)
    const uint8_t *nodeBC;                  // bit0 symmX, bit1 symmY, bit2 symmZ
    const int32_t *nodeElemStart, *cornerList;
@@ -86,6 +104,8 @@ struct DevState {
    double *dtOut;           // [0] dtcourant, [1] dthydro
    const double *dtDevice;  // device-side deltatime (null: use argument)
    const int32_t *active;   // device-side loop flag (null: always active)
+   Brick brick;             // valid when isBrick
+   int32_t isBrick;
 };
 
 } // namespace lgpu

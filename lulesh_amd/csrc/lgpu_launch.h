@@ -22,6 +22,7 @@ struct Launch {
                      hipStream_t st);
    static void node(const DevState &s, const Consts &k, int64_t planeStride, double dt, int block,
                     hipStream_t st);
+   static void nodal_brick(const DevState &s, const Consts &k, double dt, hipStream_t st);
    static void kin(const DevState &s, const Consts &k, double dt, int block, hipStream_t st);
    static void eos(const DevState &s, const Consts &k, hipStream_t st);
    static void dt_reduce(const DevState &s, hipStream_t st);

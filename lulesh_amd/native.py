@@ -23,7 +23,8 @@ FIELDS = [
 ]
 NODE_FIELDS = set(FIELDS[:13])
 FP64, MIXED = 0, 1
-OPT_KEEP_DIAGNOSTICS, OPT_KERNEL_TIMING, OPT_BLOCK_SIZE = 1, 2, 3
+OPT_KEEP_DIAGNOSTICS, OPT_KERNEL_TIMING, OPT_BLOCK_SIZE, OPT_BRICK_CHUNK, OPT_XCD_REMAP = 1, 2, 3, 4, 5
+FLAG_GENERIC = 1
 STATUS = {0: "ok", -1: "VolumeError", -2: "QStopError", -100: "EINVAL", -101: "EDEVICE",
           -102: "ENOMEM"}
 
@@ -51,6 +52,7 @@ class MeshDesc(ctypes.Structure):
         ("numReg", ctypes.c_int32),
         ("regElemSize", _i32p), ("regElemlist", ctypes.POINTER(_i32p)),
         ("cost", ctypes.c_int32), ("precision", ctypes.c_int32),
+        ("flags", ctypes.c_int32),
         ("k", Constants),
     ]
 
@@ -181,7 +183,7 @@ class DeviceContext:
 
     def __init__(self, *, nodelist, lxim, lxip, letam, letap, lzetam, lzetap, elemBC,
                  nodeElemStart, nodeElemCornerList, symmX, symmY, symmZ, regions, cost,
-                 size, precision=FP64, constants: Constants | None = None):
+                 size, precision=FP64, constants: Constants | None = None, flags: int = 0):
         lib = gpu_lib()
         self._keep = [_i32(a) for a in (nodelist, lxim, lxip, letam, letap, lzetam, lzetap,
                                         elemBC, nodeElemStart, nodeElemCornerList, symmX,
@@ -206,6 +208,7 @@ class DeviceContext:
         d.regElemlist = self._regptrs
         d.cost = cost
         d.precision = precision
+        d.flags = flags
         d.k = constants if constants is not None else default_constants()
         self.numElem, self.numNode = d.numElem, d.numNode
         self.precision = precision

@@ -32,18 +32,25 @@ def advance(o: OracleDomain, cycles: int) -> None:
         assert o.call("lagrange_leapfrog") == 0
 
 
+PATHS = [pytest.param(False, id="brick"), pytest.param(True, id="generic")]
+
+
+@pytest.mark.parametrize("generic", PATHS)
 @pytest.mark.parametrize("s,r,b,c,warm,zero_ss", [
     (5, 11, 1, 1, 0, False),   # first cycle: ss = 0, no hourglass force
     (6, 21, 2, 2, 4, True),    # later cycle with ss zeroed on both sides
     (8, 11, 1, 1, 12, False),  # later cycle with hourglass forces (cbrt)
+    (20, 11, 1, 1, 6, True),   # several brick tiles per edge, several z-chunks
+    (31, 11, 1, 1, 3, False),
 ])
-def test_lagrange_nodal(s, r, b, c, warm, zero_ss):
+def test_lagrange_nodal(s, r, b, c, warm, zero_ss, generic):
     o = OracleDomain(s, r, b, c)
     advance(o, warm)
     o.call("time_increment")
     if zero_ss:
         o.real("ss")[:] = 0.0
-    ctx = context_from_oracle(o)
+    ctx = context_from_oracle(o, generic=generic)
+    assert ctx.info().path == (0 if generic else 1)
     ctx.set_option(1, 1)  # keep fx.., xdd..
     upload_state(ctx, o)
     dt = o.get("deltatime")
@@ -58,14 +65,15 @@ def test_lagrange_nodal(s, r, b, c, warm, zero_ss):
             assert max_rel(got, want) <= 1e-12, (f, max_rel(got, want))
 
 
+@pytest.mark.parametrize("generic", PATHS)
 @pytest.mark.parametrize("s,r,b,c,warm", [(5, 11, 1, 1, 0), (6, 21, 2, 2, 3), (8, 11, 1, 1, 12),
                                           (10, 7, 2, 1, 30)])
-def test_lagrange_elements_and_dt(s, r, b, c, warm):
+def test_lagrange_elements_and_dt(s, r, b, c, warm, generic):
     o = OracleDomain(s, r, b, c)
     advance(o, warm)
     o.call("time_increment")
     assert o.call("lagrange_nodal") == 0
-    ctx = context_from_oracle(o)
+    ctx = context_from_oracle(o, generic=generic)
     ctx.set_option(1, 1)
     upload_state(ctx, o)
     dt = o.get("deltatime")
@@ -79,6 +87,27 @@ def test_lagrange_elements_and_dt(s, r, b, c, warm):
         assert np.array_equal(got, want), (f, max_rel(got, want))
     assert dtc == o.get("dtcourant")
     assert dth == o.get("dthydro")
+
+
+@pytest.mark.parametrize("s,chunk", [(7, 8), (16, 3), (33, 5)])
+def test_brick_and_generic_paths_bit_identical(s, chunk):
+    """Both nodal paths sum corners in the reference's order: whole cycles
+    agree bit for bit, whatever the brick kernel's z-chunking."""
+    o = OracleDomain(s, 11, 1, 1)
+    ctxs = [context_from_oracle(o, generic=g) for g in (False, True)]
+    ctxs[0].set_option(4, chunk)
+    for ctx in ctxs:
+        upload_state(ctx, o)
+    dt = o.get("deltatime")
+    for _ in range(12):
+        for ctx in ctxs:
+            assert ctx.lagrange_nodal(dt) == 0
+            assert ctx.lagrange_elements(dt) == 0
+        (st0, c0, h0), (st1, c1, h1) = (ctx.time_constraints() for ctx in ctxs)
+        assert st0 == st1 == 0 and c0 == c1 and h0 == h1
+        dt = min(c0 / 2, dt * 1.1)
+    for f in STATE_FIELDS:
+        assert np.array_equal(ctxs[0].download(f), ctxs[1].download(f)), f
 
 
 def test_full_cycles_match_oracle_closely():
