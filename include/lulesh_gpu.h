@@ -181,6 +181,53 @@ int lgpu_get_info(lgpu_ctx *ctx, lgpu_info *info);
 /* Last HIP error text of this context (empty if none). */
 const char *lgpu_last_error(lgpu_ctx *ctx);
 
+/* ------------------------------------------------------------------------
+ * Ensemble: a batch of independent problems advanced together.
+ *
+ * No counterpart in the reference, which runs one Domain per process; this
+ * is the reference's main loop (lulesh.cc:2745-2748) applied to many
+ * Domains at once.  Each member is described as for lgpu_create (all members
+ * share precision and constants) and keeps its own time state: TimeIncrement
+ * (lulesh.cc:166-222) runs per member on the device, so members of different
+ * sizes advance with their own dt.  One cycle of every member is captured as
+ * one hipGraph and replayed.  Results of each member equal a single-problem
+ * run of that member (the generic CSR-gather kernels are used).
+ * ---------------------------------------------------------------------- */
+typedef struct lgpu_ensemble lgpu_ensemble;
+
+int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_ensemble **out);
+void lgpu_ensemble_destroy(lgpu_ensemble *ens);
+int lgpu_ensemble_upload(lgpu_ensemble *ens, int32_t member, int field, const double *host);
+int lgpu_ensemble_download(lgpu_ensemble *ens, int32_t member, int field, double *host);
+
+/* Runs every member until its own loop test fails (time >= stoptime or
+ * cycle >= max_cycle) or its step fails.  ts holds nMembers time states and
+ * is updated; status (optional, nMembers) receives each member's status.
+ * Returns the first member error, or 0. */
+int lgpu_ensemble_run(lgpu_ensemble *ens, lgpu_time_state *ts, int32_t max_cycle, int32_t *status);
+
+enum lgpu_ensemble_option {
+   LGPU_ENS_OPT_GRAPH = 1,        /* 1 (default): replay a captured cycle graph */
+   LGPU_ENS_OPT_TIMING = 2,       /* time every kernel (no graph while on)      */
+   LGPU_ENS_OPT_BLOCK_FORCE = 3,  /* threads per block: 64, 128 or 256          */
+   LGPU_ENS_OPT_BLOCK_NODE = 4,
+   LGPU_ENS_OPT_BLOCK_KIN = 5,
+   LGPU_ENS_OPT_XCD_REMAP = 6
+};
+int lgpu_ensemble_set_option(lgpu_ensemble *ens, int option, int64_t value);
+int lgpu_ensemble_get_option(lgpu_ensemble *ens, int option, int64_t *value);
+
+/* Autotuner: times trial cycles of this ensemble for each candidate block
+ * shape of the force, node and kinematics kernels and for both XCD block
+ * mappings, keeps the fastest, and restores the state afterwards. */
+int lgpu_ensemble_autotune(lgpu_ensemble *ens, const lgpu_time_state *ts, int32_t trial_cycles);
+
+int lgpu_ensemble_num_kernels(void);
+const char *lgpu_ensemble_kernel_name(int kernel);
+int lgpu_ensemble_kernel_stats(lgpu_ensemble *ens, int kernel, int64_t *launches, double *total_ms);
+int64_t lgpu_ensemble_num_elem(lgpu_ensemble *ens);
+const char *lgpu_ensemble_last_error(lgpu_ensemble *ens);
+
 #ifdef __cplusplus
 }
 #endif

@@ -62,6 +62,36 @@ void lh_symmetry(lh_domain *d, double out[3]);
  * own generator (the source of the region layout, lulesh-init.cc:406-480). */
 void lh_glibc_rand(uint32_t seed, int32_t n, int32_t *out);
 
+/* ------------------------------------------------------------------------
+ * Ensembles of independent Sedov problems (device side: lgpu_ensemble).
+ * No counterpart in the reference (one Domain per process there).
+ * ---------------------------------------------------------------------- */
+typedef struct lh_ensemble lh_ensemble;
+
+/* Member parameters drawn from std::mt19937_64(seed), four raw 64-bit
+ * outputs per member, in order: edge s = 16 + x0 mod 49 (16..64); energy
+ * multiplier exp(ln 0.5 + ln 4 * u), u = (x1 >> 11) * 2^-53 (log-uniform in
+ * [0.5, 2)); balance = 1 + x2 mod 2; cost = 1 + x3 mod 2. */
+void lh_ensemble_draw(uint64_t seed, int32_t n, int32_t *s, double *emult, int32_t *balance,
+                      int32_t *cost);
+
+/* Builds n host Domains (nr[i] regions each) and their device ensemble. */
+lh_ensemble *lh_ensemble_create(int32_t n, const int32_t *s, const double *emult, const int32_t *nr,
+                                const int32_t *balance, const int32_t *cost, int32_t precision);
+void lh_ensemble_destroy(lh_ensemble *e);
+
+/* Runs every member's main loop to cycle its (or t_stop); status optional
+ * (n entries).  Returns 0 or the first member error. */
+int lh_ensemble_run(lh_ensemble *e, int32_t its, int32_t *status);
+
+/* Copies every member's state back into its host Domain. */
+int lh_ensemble_sync(lh_ensemble *e);
+
+/* Host Domain of member m, owned by the ensemble (do not lh_destroy it). */
+lh_domain *lh_ensemble_member(lh_ensemble *e, int32_t m);
+lgpu_ensemble *lh_ensemble_device(lh_ensemble *e);
+int32_t lh_ensemble_size(lh_ensemble *e);
+
 #ifdef __cplusplus
 }
 #endif

@@ -37,6 +37,7 @@ HIP_SOURCES = {
     "lgpu_api.hip": ["-ffp-contract=off"],
     "lgpu_step_fp64.hip": ["-ffp-contract=off"],
     "lgpu_step_mixed.hip": ["-ffp-contract=fast"],
+    "lgpu_ensemble.hip": ["-ffp-contract=off"],
 }
 HOST_SOURCES = ["host/domain.cc", "host/glibc_rand.cc"]
 

@@ -7,6 +7,8 @@
 #include <stdlib.h>
 
 #include <algorithm>
+#include <memory>
+#include <random>
 
 #include "host/glibc_rand.h"
 #include "lulesh_host.h"
@@ -412,21 +414,92 @@ int LagrangeLeapFrogStatus(Domain &domain)
 
 struct lh_domain {
    Domain *d;
+   bool owned;
 };
+
+struct lh_ensemble {
+   std::vector<std::unique_ptr<Domain>> members;
+   std::vector<lh_domain> views;
+   lgpu_ensemble *dev = nullptr;
+};
+
+namespace {
+
+// The lgpu_mesh_desc of a host Domain (arrays borrowed from it; the region
+// pointer table lives in *lists).
+lgpu_mesh_desc describe(Domain &d, std::vector<const Index_t *> &lists, int precision)
+{
+   lists.resize((size_t)d.numReg());
+   for (Int_t r = 0; r < d.numReg(); ++r) lists[(size_t)r] = d.regElemlist(r);
+   lgpu_mesh_desc m{};
+   m.numElem = d.numElem();
+   m.numNode = d.numNode();
+   m.sizeX = d.sizeX();
+   m.sizeY = d.sizeY();
+   m.sizeZ = d.sizeZ();
+   m.nodelist = d.nodelist(0);
+   m.lxim = &d.lxim(0);
+   m.lxip = &d.lxip(0);
+   m.letam = &d.letam(0);
+   m.letap = &d.letap(0);
+   m.lzetam = &d.lzetam(0);
+   m.lzetap = &d.lzetap(0);
+   m.elemBC = &d.elemBC(0);
+   m.nodeElemStart = d.nodeElemStartVec().data();
+   m.nodeElemCornerList = d.cornerListVec().data();
+   m.symmX = d.symmXdata();
+   m.symmY = d.symmYdata();
+   m.symmZ = d.symmZdata();
+   m.numSymmX = m.numSymmY = m.numSymmZ = (d.sizeX() + 1) * (d.sizeX() + 1);
+   m.numReg = d.numReg();
+   m.regElemSize = &d.regElemSize(0);
+   m.regElemlist = lists.data();
+   m.cost = d.cost();
+   m.precision = precision;
+   m.k = d.constants();
+   return m;
+}
+
+lgpu_time_state time_state(Domain &d)
+{
+   lgpu_time_state ts;
+   ts.time = d.time();
+   ts.deltatime = d.deltatime();
+   ts.dtcourant = d.dtcourant();
+   ts.dthydro = d.dthydro();
+   ts.stoptime = d.stoptime();
+   ts.dtfixed = d.dtfixed();
+   ts.dtmax = d.dtmax();
+   ts.deltatimemultlb = d.deltatimemultlb();
+   ts.deltatimemultub = d.deltatimemultub();
+   ts.cycle = d.cycle();
+   return ts;
+}
+
+void set_time_state(Domain &d, const lgpu_time_state &ts)
+{
+   d.time() = ts.time;
+   d.deltatime() = ts.deltatime;
+   d.dtcourant() = ts.dtcourant;
+   d.dthydro() = ts.dthydro;
+   d.cycle() = ts.cycle;
+}
+
+} // namespace
 
 extern "C" {
 
 lh_domain *lh_create(int32_t s, int32_t nr, int32_t balance, int32_t cost, double emult)
 {
    if (s < 1 || nr < 1 || cost < 0) return nullptr;
-   lh_domain *h = new lh_domain{new Domain(1, 0, 0, 0, s, 1, nr, balance, cost)};
+   lh_domain *h = new lh_domain{new Domain(1, 0, 0, 0, s, 1, nr, balance, cost), true};
    if (emult != 1.0) h->d->ScaleInitialEnergy(emult);
    return h;
 }
 
 void lh_destroy(lh_domain *h)
 {
-   if (!h) return;
+   if (!h || !h->owned) return;
    delete h->d;
    delete h;
 }
@@ -576,5 +649,88 @@ void lh_glibc_rand(uint32_t seed, int32_t n, int32_t *out)
    lhost::GlibcRand g(seed);
    for (int32_t i = 0; i < n; ++i) out[i] = g.next();
 }
+
+void lh_ensemble_draw(uint64_t seed, int32_t n, int32_t *s, double *emult, int32_t *balance,
+                      int32_t *cost)
+{
+   std::mt19937_64 g(seed);
+   const double lo = log(0.5), span = log(2.0) - log(0.5);
+   for (int32_t i = 0; i < n; ++i) {
+      const uint64_t x0 = g(), x1 = g(), x2 = g(), x3 = g();
+      if (s) s[i] = 16 + (int32_t)(x0 % 49u);
+      if (emult) emult[i] = exp(lo + span * ((double)(x1 >> 11) * 0x1.0p-53));
+      if (balance) balance[i] = 1 + (int32_t)(x2 % 2u);
+      if (cost) cost[i] = 1 + (int32_t)(x3 % 2u);
+   }
+}
+
+lh_ensemble *lh_ensemble_create(int32_t n, const int32_t *s, const double *emult, const int32_t *nr,
+                                const int32_t *balance, const int32_t *cost, int32_t precision)
+{
+   if (n < 1 || !s || !emult || !nr || !balance || !cost) return nullptr;
+   for (int32_t i = 0; i < n; ++i)
+      if (s[i] < 1 || nr[i] < 1 || cost[i] < 0) return nullptr;
+   auto *e = new lh_ensemble();
+   for (int32_t i = 0; i < n; ++i) {
+      e->members.emplace_back(new Domain(1, 0, 0, 0, s[i], 1, nr[i], balance[i], cost[i]));
+      if (emult[i] != 1.0) e->members.back()->ScaleInitialEnergy(emult[i]);
+   }
+   for (auto &m : e->members) e->views.push_back(lh_domain{m.get(), false});
+   std::vector<std::vector<const Index_t *>> lists((size_t)n);
+   std::vector<lgpu_mesh_desc> descs;
+   for (int32_t i = 0; i < n; ++i) descs.push_back(describe(*e->members[(size_t)i], lists[(size_t)i], precision));
+   if (lgpu_ensemble_create(descs.data(), n, &e->dev) != LGPU_OK) {
+      delete e;
+      return nullptr;
+   }
+   static const int kUp[] = {LGPU_X, LGPU_Y, LGPU_Z, LGPU_XD, LGPU_YD, LGPU_ZD, LGPU_NODALMASS,
+                             LGPU_E, LGPU_P, LGPU_Q, LGPU_V, LGPU_VOLO, LGPU_SS, LGPU_ELEMMASS};
+   for (int32_t i = 0; i < n; ++i)
+      for (int f : kUp)
+         if (lgpu_ensemble_upload(e->dev, i, f, e->members[(size_t)i]->realField(f)) != LGPU_OK) {
+            lh_ensemble_destroy(e);
+            return nullptr;
+         }
+   return e;
+}
+
+void lh_ensemble_destroy(lh_ensemble *e)
+{
+   if (!e) return;
+   if (e->dev) lgpu_ensemble_destroy(e->dev);
+   delete e;
+}
+
+int lh_ensemble_run(lh_ensemble *e, int32_t its, int32_t *status)
+{
+   if (!e) return LGPU_EINVAL;
+   std::vector<lgpu_time_state> ts;
+   for (auto &m : e->members) ts.push_back(time_state(*m));
+   const int st = lgpu_ensemble_run(e->dev, ts.data(), its, status);
+   for (size_t i = 0; i < ts.size(); ++i) set_time_state(*e->members[i], ts[i]);
+   return st;
+}
+
+int lh_ensemble_sync(lh_ensemble *e)
+{
+   if (!e) return LGPU_EINVAL;
+   static const int kDown[] = {LGPU_X, LGPU_Y, LGPU_Z, LGPU_XD, LGPU_YD, LGPU_ZD, LGPU_E, LGPU_P,
+                               LGPU_Q, LGPU_V, LGPU_SS, LGPU_VNEW, LGPU_DELV, LGPU_VDOV, LGPU_AREALG};
+   for (size_t i = 0; i < e->members.size(); ++i)
+      for (int f : kDown) {
+         const int st = lgpu_ensemble_download(e->dev, (int32_t)i, f, e->members[i]->realField(f));
+         if (st) return st;
+      }
+   return LGPU_OK;
+}
+
+lh_domain *lh_ensemble_member(lh_ensemble *e, int32_t m)
+{
+   if (!e || m < 0 || (size_t)m >= e->views.size()) return nullptr;
+   return &e->views[(size_t)m];
+}
+
+lgpu_ensemble *lh_ensemble_device(lh_ensemble *e) { return e ? e->dev : nullptr; }
+int32_t lh_ensemble_size(lh_ensemble *e) { return e ? (int32_t)e->members.size() : 0; }
 
 } // extern "C"

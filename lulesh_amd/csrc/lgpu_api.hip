@@ -508,6 +508,7 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    s.numElem = d->numElem;
    s.numNode = d->numNode;
    s.numReg = d->numReg;
+   s.numPerm = d->numElem;
    s.xcdRemap = 1;
    s.isBrick = brick ? 1 : 0;
    if (brick) {

@@ -1,0 +1,735 @@
+// lgpu_ensemble.hip -- a batch of independent Sedov problems advanced
+// together: one launch per kernel covers every member, each member keeps its
+// own dt (TimeIncrement, lulesh.cc:166-222, evaluated per member on the
+// device), and one cycle is captured once as a hipGraph and replayed.
+//
+// Members are concatenated in one device arena: node arrays, element arrays
+// and node-to-corner lists are offset per member, and the region-ordered EOS
+// list is padded per member to whole k_eos blocks so that every block (and
+// so every dt partial) belongs to exactly one member.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "lgpu_kernels.h"
+#include "lgpu_launch.h"
+#include "lulesh_gpu.h"
+
+using namespace lgpu;
+
+namespace {
+
+enum EnsKernel { kEForce = 0, kENode, kEKin, kEEos, kEDt, kETime, kENumKernels };
+const char *const kEnsKernelNames[kENumKernels] = {"k_force", "k_node", "k_kin", "k_eos",
+                                                   "k_dt_reduce", "k_time_increment"};
+
+constexpr size_t kAlign = 256;
+inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
+
+// Per-member time state, struct of arrays (the kernels read deltatime and
+// active as plain arrays).
+struct EnsTime {
+   double *time, *deltatime, *dtcourant, *dthydro, *stoptime, *dtfixed, *dtmax, *multlb, *multub;
+   int32_t *cycle, *active;
+};
+
+// TimeIncrement per member (lulesh.cc:166-222) and the loop test of
+// lulesh.cc:2745; a member whose step failed stops.
+__global__ void k_time_increment_ens(EnsTime t, const double *dtOut, const int32_t *err,
+                                     int nMembers, int maxCycle)
+{
+   const int m = blockIdx.x * blockDim.x + threadIdx.x;
+   if (m >= nMembers || !t.active[m]) return;
+   bool failed = false;
+   for (int i = 0; i < kNumErrSites; ++i) failed = failed || (err[m * kNumErrSites + i] != 0);
+   if (t.cycle[m] > 0) {
+      t.dtcourant[m] = dtOut[2 * m];
+      t.dthydro[m] = dtOut[2 * m + 1];
+   }
+   if (failed || !(t.time[m] < t.stoptime[m] && t.cycle[m] < maxCycle)) {
+      t.active[m] = 0;
+      return;
+   }
+   double targetdt = t.stoptime[m] - t.time[m];
+   double dt = t.deltatime[m];
+   if (t.dtfixed[m] <= 0.0 && t.cycle[m] != 0) {
+      const double olddt = dt;
+      double newdt = 1.0e+20;
+      if (t.dtcourant[m] < newdt) newdt = t.dtcourant[m] / 2.0;
+      if (t.dthydro[m] < newdt) newdt = t.dthydro[m] * 2.0 / 3.0;
+      const double ratio = newdt / olddt;
+      if (ratio >= 1.0) {
+         if (ratio < t.multlb[m]) newdt = olddt;
+         else if (ratio > t.multub[m]) newdt = olddt * t.multub[m];
+      }
+      if (newdt > t.dtmax[m]) newdt = t.dtmax[m];
+      dt = newdt;
+   }
+   if (targetdt > dt && targetdt < 4.0 * dt / 3.0) targetdt = 2.0 * dt / 3.0;
+   if (targetdt < dt) dt = targetdt;
+   t.deltatime[m] = dt;
+   t.time[m] += dt;
+   t.cycle[m] += 1;
+}
+
+int32_t region_rep(int32_t r, int32_t numReg, int32_t cost)
+{
+   if (r < numReg / 2) return 1;
+   if (r < numReg - (numReg + 15) / 20) return 1 + cost;
+   return 10 * (1 + cost);
+}
+
+} // namespace
+
+struct lgpu_ensemble {
+   hipStream_t stream = nullptr;
+   int32_t nMembers = 0, precision = 0;
+   Consts k{};
+   DevState s{};
+   int64_t planeStride = 0;
+   char *arena = nullptr;
+   size_t arenaBytes = 0;
+   size_t stateBytes = 0; // leading part of the arena holding the state (for snapshots)
+   std::vector<int32_t> nodeStart, elemStart, nodeCount, elemCount;
+   EnsTime t{};
+   int block[3] = {256, 256, 256}; // force, node, kin
+   bool useGraph = true;
+   bool timing = false;
+   hipGraph_t graph = nullptr;
+   hipGraphExec_t exec = nullptr;
+   int graphMaxCycle = -1;
+   int64_t kLaunches[kENumKernels] = {};
+   double kMs[kENumKernels] = {};
+   std::string lastError;
+   // pinned host mirror of the time state
+   char *host = nullptr;
+};
+
+namespace {
+
+int fail_hip(lgpu_ensemble *e, hipError_t err, const char *what)
+{
+   if (err == hipSuccess) return LGPU_OK;
+   if (e) e->lastError = std::string(what) + ": " + hipGetErrorString(err);
+   return err == hipErrorOutOfMemory ? LGPU_ENOMEM : LGPU_EDEVICE;
+}
+
+#define ENS_HIP(e, call)                                            \
+   do {                                                             \
+      const hipError_t err_ = (call);                               \
+      if (err_ != hipSuccess) return fail_hip((e), err_, #call);    \
+   } while (0)
+
+template <class P>
+void launch_cycle(lgpu_ensemble *e, int maxCycle)
+{
+   hipLaunchKernelGGL(k_time_increment_ens, dim3((e->nMembers + 63) / 64), dim3(64), 0, e->stream,
+                      e->t, e->s.memberDtOut, e->s.memberErr, e->nMembers, maxCycle);
+   Launch<P>::force(e->s, e->k, e->planeStride, e->block[0], e->stream);
+   Launch<P>::node(e->s, e->k, e->planeStride, 0.0, e->block[1], e->stream);
+   Launch<P>::kin(e->s, e->k, 0.0, e->block[2], e->stream);
+   Launch<P>::eos(e->s, e->k, e->stream);
+   Launch<P>::dt_reduce(e->s, e->stream);
+}
+
+void enqueue_cycle(lgpu_ensemble *e, int maxCycle)
+{
+   if (e->precision == LGPU_MIXED) launch_cycle<PolicyMixed>(e, maxCycle);
+   else launch_cycle<PolicyFp64>(e, maxCycle);
+}
+
+// Per-kernel timing of one cycle (used by the autotuner and the timing
+// option): events around each launch.
+template <class P>
+int timed_cycle(lgpu_ensemble *e, int maxCycle, float ms[kENumKernels])
+{
+   hipEvent_t ev[kENumKernels + 1];
+   for (auto &x : ev) ENS_HIP(e, hipEventCreate(&x));
+   ENS_HIP(e, hipEventRecord(ev[0], e->stream));
+   hipLaunchKernelGGL(k_time_increment_ens, dim3((e->nMembers + 63) / 64), dim3(64), 0, e->stream,
+                      e->t, e->s.memberDtOut, e->s.memberErr, e->nMembers, maxCycle);
+   ENS_HIP(e, hipEventRecord(ev[1], e->stream));
+   Launch<P>::force(e->s, e->k, e->planeStride, e->block[0], e->stream);
+   ENS_HIP(e, hipEventRecord(ev[2], e->stream));
+   Launch<P>::node(e->s, e->k, e->planeStride, 0.0, e->block[1], e->stream);
+   ENS_HIP(e, hipEventRecord(ev[3], e->stream));
+   Launch<P>::kin(e->s, e->k, 0.0, e->block[2], e->stream);
+   ENS_HIP(e, hipEventRecord(ev[4], e->stream));
+   Launch<P>::eos(e->s, e->k, e->stream);
+   ENS_HIP(e, hipEventRecord(ev[5], e->stream));
+   Launch<P>::dt_reduce(e->s, e->stream);
+   ENS_HIP(e, hipEventRecord(ev[6], e->stream));
+   ENS_HIP(e, hipGetLastError());
+   ENS_HIP(e, hipStreamSynchronize(e->stream));
+   // ev[i]..ev[i+1] brackets: 0 time, 1 force, 2 node, 3 kin, 4 eos, 5 dt
+   const int order[6] = {kETime, kEForce, kENode, kEKin, kEEos, kEDt};
+   for (int i = 0; i < 6; ++i) ENS_HIP(e, hipEventElapsedTime(&ms[order[i]], ev[i], ev[i + 1]));
+   for (auto &x : ev) (void)hipEventDestroy(x);
+   return LGPU_OK;
+}
+
+int timed_cycle_any(lgpu_ensemble *e, int maxCycle, float ms[kENumKernels])
+{
+   return e->precision == LGPU_MIXED ? timed_cycle<PolicyMixed>(e, maxCycle, ms)
+                                     : timed_cycle<PolicyFp64>(e, maxCycle, ms);
+}
+
+void drop_graph(lgpu_ensemble *e)
+{
+   if (e->exec) (void)hipGraphExecDestroy(e->exec);
+   if (e->graph) (void)hipGraphDestroy(e->graph);
+   e->exec = nullptr;
+   e->graph = nullptr;
+   e->graphMaxCycle = -1;
+}
+
+int build_graph(lgpu_ensemble *e, int maxCycle)
+{
+   drop_graph(e);
+   ENS_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+   enqueue_cycle(e, maxCycle);
+   hipGraph_t g = nullptr;
+   const hipError_t endErr = hipStreamEndCapture(e->stream, &g);
+   ENS_HIP(e, endErr);
+   e->graph = g;
+   ENS_HIP(e, hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
+   e->graphMaxCycle = maxCycle;
+   return LGPU_OK;
+}
+
+double *field_base(lgpu_ensemble *e, int f, bool &isElem, bool &isGrad)
+{
+   isElem = f >= LGPU_E;
+   isGrad = f >= LGPU_DELX_XI;
+   DevState &s = e->s;
+   switch (f) {
+   case LGPU_X: return s.x;
+   case LGPU_Y: return s.y;
+   case LGPU_Z: return s.z;
+   case LGPU_XD: return s.xd;
+   case LGPU_YD: return s.yd;
+   case LGPU_ZD: return s.zd;
+   case LGPU_NODALMASS: return s.nodalMass;
+   case LGPU_E: return s.e;
+   case LGPU_P: return s.p;
+   case LGPU_Q: return s.q;
+   case LGPU_V: return s.v;
+   case LGPU_VOLO: return s.volo;
+   case LGPU_VNEW: return s.vnew;
+   case LGPU_DELV: return s.delv;
+   case LGPU_VDOV: return s.vdov;
+   case LGPU_AREALG: return s.arealg;
+   case LGPU_SS: return s.ss;
+   case LGPU_ELEMMASS: return s.elemMass;
+   case LGPU_DELX_XI: return static_cast<double *>(s.delx_xi);
+   case LGPU_DELX_ETA: return static_cast<double *>(s.delx_eta);
+   case LGPU_DELX_ZETA: return static_cast<double *>(s.delx_zeta);
+   case LGPU_DELV_XI: return static_cast<double *>(s.delv_xi);
+   case LGPU_DELV_ETA: return static_cast<double *>(s.delv_eta);
+   case LGPU_DELV_ZETA: return static_cast<double *>(s.delv_zeta);
+   default: return nullptr;
+   }
+}
+
+} // namespace
+
+extern "C" {
+
+int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_ensemble **out)
+{
+   if (!members || nMembers < 1 || !out) return LGPU_EINVAL;
+   *out = nullptr;
+   const int32_t prec = members[0].precision;
+   for (int32_t m = 0; m < nMembers; ++m) {
+      const lgpu_mesh_desc &d = members[m];
+      if (d.precision != prec || memcmp(&d.k, &members[0].k, sizeof(lgpu_constants)) != 0) {
+         fprintf(stderr, "lgpu_ensemble_create: members must share precision and constants\n");
+         return LGPU_EINVAL;
+      }
+      // Validate each member on its own (connectivity ranges, regions).
+      if (d.numElem < 1 || d.numNode < 1 || !d.nodelist || !d.nodeElemStart ||
+          !d.nodeElemCornerList || d.numReg < 1 || d.cost < 0)
+         return LGPU_EINVAL;
+      for (int64_t i = 0; i < 8 * (int64_t)d.numElem; ++i)
+         if (d.nodelist[i] < 0 || d.nodelist[i] >= d.numNode) return LGPU_EINVAL;
+      const int32_t *nb[6] = {d.lxim, d.lxip, d.letam, d.letap, d.lzetam, d.lzetap};
+      for (const int32_t *a : nb) {
+         if (!a) return LGPU_EINVAL;
+         for (int32_t i = 0; i < d.numElem; ++i)
+            if (a[i] < 0 || a[i] >= d.numElem) return LGPU_EINVAL;
+      }
+      for (int32_t n = 0; n < d.numNode; ++n)
+         if (d.nodeElemStart[n + 1] < d.nodeElemStart[n]) return LGPU_EINVAL;
+      if (d.nodeElemStart[0] != 0 || d.nodeElemStart[d.numNode] > 8 * d.numElem) return LGPU_EINVAL;
+      for (int32_t c = 0; c < d.nodeElemStart[d.numNode]; ++c)
+         if (d.nodeElemCornerList[c] < 0 || d.nodeElemCornerList[c] >= 8 * d.numElem) return LGPU_EINVAL;
+      int64_t tot = 0;
+      std::vector<uint8_t> seen((size_t)d.numElem, 0);
+      for (int32_t r = 0; r < d.numReg; ++r)
+         for (int32_t i = 0; i < d.regElemSize[r]; ++i) {
+            const int32_t el = d.regElemlist[r][i];
+            if (el < 0 || el >= d.numElem || seen[(size_t)el]) return LGPU_EINVAL;
+            seen[(size_t)el] = 1;
+            ++tot;
+         }
+      if (tot != d.numElem) return LGPU_EINVAL;
+      const int32_t *sy[3] = {d.symmX, d.symmY, d.symmZ};
+      const int32_t ns[3] = {d.numSymmX, d.numSymmY, d.numSymmZ};
+      for (int a = 0; a < 3; ++a)
+         for (int32_t i = 0; i < ns[a]; ++i)
+            if (!sy[a] || sy[a][i] < 0 || sy[a][i] >= d.numNode) return LGPU_EINVAL;
+   }
+
+   lgpu_ensemble *e = new (std::nothrow) lgpu_ensemble();
+   if (!e) return LGPU_ENOMEM;
+   auto bail = [&](int code) {
+      lgpu_ensemble_destroy(e);
+      return code;
+   };
+   e->nMembers = nMembers;
+   e->precision = prec;
+   const lgpu_constants &kk = members[0].k;
+   e->k = Consts{kk.e_cut, kk.p_cut, kk.q_cut, kk.v_cut, kk.u_cut, kk.hgcoef, kk.ss4o3,
+                 kk.qstop, kk.monoq_max_slope, kk.monoq_limiter_mult, kk.qlc_monoq,
+                 kk.qqc_monoq, kk.qqc, kk.eosvmax, kk.eosvmin, kk.pmin, kk.emin, kk.dvovmax,
+                 kk.refdens};
+   if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
+      return bail(LGPU_EDEVICE);
+
+   // Offsets.
+   e->nodeStart.assign((size_t)nMembers + 1, 0);
+   e->elemStart.assign((size_t)nMembers + 1, 0);
+   std::vector<int64_t> cornerStart((size_t)nMembers + 1, 0);
+   std::vector<int32_t> blockStart((size_t)nMembers + 1, 0);
+   int64_t numRegTot = 0;
+   for (int32_t m = 0; m < nMembers; ++m) {
+      const lgpu_mesh_desc &d = members[m];
+      e->nodeStart[m + 1] = e->nodeStart[m] + d.numNode;
+      e->elemStart[m + 1] = e->elemStart[m] + d.numElem;
+      cornerStart[m + 1] = cornerStart[m] + d.nodeElemStart[d.numNode];
+      blockStart[m + 1] = blockStart[m] + (int32_t)eos_blocks(d.numElem);
+      e->nodeCount.push_back(d.numNode);
+      e->elemCount.push_back(d.numElem);
+      numRegTot += d.numReg;
+   }
+   const size_t E = (size_t)e->elemStart[nMembers], N = (size_t)e->nodeStart[nMembers];
+   const size_t nPerm = (size_t)blockStart[nMembers] * kEosBlock;
+   const size_t nCorner = (size_t)cornerStart[nMembers];
+   const size_t cb = prec == LGPU_MIXED ? 4 : 8, gb = cb;
+   const int64_t plane = (int64_t)(align_up(E * cb) / cb);
+   e->planeStride = plane;
+
+   // Arena: state first (snapshot-able), then connectivity and scratch.
+   struct Slot { void **dst; size_t bytes; };
+   std::vector<Slot> slots;
+   DevState &s = e->s;
+   auto add = [&](void **dst, size_t bytes) { slots.push_back({dst, align_up(bytes)}); };
+   double **stateNode[] = {&s.x, &s.y, &s.z, &s.xd, &s.yd, &s.zd};
+   for (double **p : stateNode) add((void **)p, 8 * N);
+   double **stateElem[] = {&s.e, &s.p, &s.q, &s.v, &s.ss};
+   for (double **p : stateElem) add((void **)p, 8 * E);
+   EnsTime &t = e->t;
+   double **tf[] = {&t.time, &t.deltatime, &t.dtcourant, &t.dthydro, &t.stoptime, &t.dtfixed,
+                    &t.dtmax, &t.multlb, &t.multub};
+   for (double **p : tf) add((void **)p, 8 * (size_t)nMembers);
+   add((void **)&t.cycle, 4 * (size_t)nMembers);
+   add((void **)&t.active, 4 * (size_t)nMembers);
+   add((void **)&s.memberErr, 4 * (size_t)nMembers * kNumErrSites);
+   add((void **)&s.memberDtOut, 16 * (size_t)nMembers);
+   const size_t stateSlots = slots.size();
+   add((void **)&s.nodalMass, 8 * N);
+   double **elemFix[] = {&s.volo, &s.elemMass, &s.vnew, &s.delv, &s.arealg, &s.vdov};
+   for (double **p : elemFix) add((void **)p, 8 * E);
+   void **grads[] = {&s.delx_xi, &s.delx_eta, &s.delx_zeta, &s.delv_xi, &s.delv_eta, &s.delv_zeta};
+   for (void **p : grads) add(p, gb * E);
+   add((void **)&s.nodeBC, N);
+   add((void **)&s.nodeElemStart, 4 * (N + 1));
+   add((void **)&s.cornerList, 4 * std::max<size_t>(nCorner, 1));
+   add((void **)&s.nodelist, 32 * E);
+   const int32_t **nbr[] = {&s.lxim, &s.lxip, &s.letam, &s.letap, &s.lzetam, &s.lzetap, &s.elemBC};
+   for (const int32_t **p : nbr) add((void **)p, 4 * E);
+   add(&s.corner, cb * (size_t)plane * 48);
+   add((void **)&s.perm, 4 * nPerm);
+   add((void **)&s.regStart, 4 * ((size_t)numRegTot + 1));
+   add((void **)&s.regRep, 4 * (size_t)numRegTot);
+   add((void **)&s.err, 4 * kNumErrSites);
+   add((void **)&s.dtPartial, 16 * (size_t)blockStart[nMembers]);
+   add((void **)&s.dtOut, 16);
+   add((void **)&s.memberNodeStart, 4 * ((size_t)nMembers + 1));
+   add((void **)&s.memberElemStart, 4 * ((size_t)nMembers + 1));
+   add((void **)&s.blockMember, 4 * (size_t)blockStart[nMembers]);
+   add((void **)&s.memberBlockStart, 4 * ((size_t)nMembers + 1));
+   size_t total = 0, stateBytes = 0;
+   for (size_t i = 0; i < slots.size(); ++i) {
+      if (i == stateSlots) stateBytes = total;
+      total += slots[i].bytes;
+   }
+   if (hipMalloc((void **)&e->arena, total) != hipSuccess) {
+      (void)hipGetLastError();
+      return bail(LGPU_ENOMEM);
+   }
+   e->arenaBytes = total;
+   e->stateBytes = stateBytes;
+   if (hipMemset(e->arena, 0, total) != hipSuccess) return bail(LGPU_EDEVICE);
+   size_t off = 0;
+   for (const Slot &sl : slots) {
+      *sl.dst = e->arena + off;
+      off += sl.bytes;
+   }
+   s.numElem = (int32_t)E;
+   s.numNode = (int32_t)N;
+   s.numReg = (int32_t)numRegTot;
+   s.numPerm = (int32_t)nPerm;
+   s.xcdRemap = 1;
+   s.isBrick = 0;
+   s.nMembers = nMembers;
+   s.memberDt = t.deltatime;
+   s.memberActive = t.active;
+
+   // Concatenated connectivity.
+   std::vector<int32_t> nodelist(8 * E), nbrs[7], nodeElemStart(N + 1), corner(nCorner), perm(nPerm, -1);
+   for (auto &v : nbrs) v.assign(E, 0);
+   std::vector<int32_t> regStart((size_t)numRegTot + 1, 0), regRep((size_t)numRegTot, 1);
+   std::vector<int32_t> blockMember((size_t)blockStart[nMembers], -1);
+   std::vector<uint8_t> bc(N, 0);
+   int64_t regBase = 0;
+   for (int32_t m = 0; m < nMembers; ++m) {
+      const lgpu_mesh_desc &d = members[m];
+      const int32_t n0 = e->nodeStart[m], e0 = e->elemStart[m];
+      for (int64_t i = 0; i < 8 * (int64_t)d.numElem; ++i) nodelist[8 * (size_t)e0 + i] = d.nodelist[i] + n0;
+      const int32_t *src[7] = {d.lxim, d.lxip, d.letam, d.letap, d.lzetam, d.lzetap, d.elemBC};
+      for (int a = 0; a < 7; ++a)
+         for (int32_t i = 0; i < d.numElem; ++i)
+            nbrs[a][(size_t)(e0 + i)] = (a < 6) ? src[a][i] + e0 : src[a][i];
+      for (int32_t n = 0; n <= d.numNode; ++n)
+         nodeElemStart[(size_t)(n0 + n)] = (int32_t)(cornerStart[m] + d.nodeElemStart[n]);
+      for (int32_t c = 0; c < d.nodeElemStart[d.numNode]; ++c)
+         corner[(size_t)(cornerStart[m] + c)] = d.nodeElemCornerList[c] + 8 * e0;
+      const int32_t *sy[3] = {d.symmX, d.symmY, d.symmZ};
+      const int32_t ns[3] = {d.numSymmX, d.numSymmY, d.numSymmZ};
+      for (int a = 0; a < 3; ++a)
+         for (int32_t i = 0; i < ns[a]; ++i) bc[(size_t)(n0 + sy[a][i])] |= (uint8_t)(1u << a);
+      int64_t pos = (int64_t)blockStart[m] * kEosBlock;
+      for (int32_t r = 0; r < d.numReg; ++r) {
+         regStart[(size_t)(regBase + r)] = (int32_t)pos;
+         regRep[(size_t)(regBase + r)] = region_rep(r, d.numReg, d.cost);
+         for (int32_t i = 0; i < d.regElemSize[r]; ++i) perm[(size_t)pos++] = d.regElemlist[r][i] + e0;
+      }
+      regBase += d.numReg;
+      for (int32_t b = blockStart[m]; b < blockStart[m + 1]; ++b) blockMember[(size_t)b] = m;
+   }
+   regStart[(size_t)numRegTot] = (int32_t)nPerm;
+
+   auto h2d = [&](void *dst, const void *src, size_t bytes) {
+      return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
+   };
+   std::vector<double> ones(E, 1.0);
+   bool ok = h2d((void *)s.nodelist, nodelist.data(), 4 * nodelist.size()) &&
+             h2d((void *)s.lxim, nbrs[0].data(), 4 * E) && h2d((void *)s.lxip, nbrs[1].data(), 4 * E) &&
+             h2d((void *)s.letam, nbrs[2].data(), 4 * E) && h2d((void *)s.letap, nbrs[3].data(), 4 * E) &&
+             h2d((void *)s.lzetam, nbrs[4].data(), 4 * E) && h2d((void *)s.lzetap, nbrs[5].data(), 4 * E) &&
+             h2d((void *)s.elemBC, nbrs[6].data(), 4 * E) &&
+             h2d((void *)s.nodeElemStart, nodeElemStart.data(), 4 * (N + 1)) &&
+             (nCorner == 0 || h2d((void *)s.cornerList, corner.data(), 4 * nCorner)) &&
+             h2d((void *)s.perm, perm.data(), 4 * nPerm) &&
+             h2d((void *)s.regStart, regStart.data(), 4 * regStart.size()) &&
+             h2d((void *)s.regRep, regRep.data(), 4 * regRep.size()) &&
+             h2d((void *)s.nodeBC, bc.data(), N) &&
+             h2d((void *)s.memberNodeStart, e->nodeStart.data(), 4 * e->nodeStart.size()) &&
+             h2d((void *)s.memberElemStart, e->elemStart.data(), 4 * e->elemStart.size()) &&
+             h2d((void *)s.blockMember, blockMember.data(), 4 * blockMember.size()) &&
+             h2d((void *)s.memberBlockStart, blockStart.data(), 4 * blockStart.size()) &&
+             h2d(s.v, ones.data(), 8 * E);
+   if (!ok) return bail(LGPU_EDEVICE);
+   if (hipHostMalloc((void **)&e->host, (size_t)nMembers * (9 * 8 + 8) + 64 +
+                                            (size_t)nMembers * kNumErrSites * 4,
+                     hipHostMallocDefault) != hipSuccess)
+      return bail(LGPU_ENOMEM);
+   *out = e;
+   return LGPU_OK;
+}
+
+void lgpu_ensemble_destroy(lgpu_ensemble *e)
+{
+   if (!e) return;
+   if (e->stream) (void)hipStreamSynchronize(e->stream);
+   drop_graph(e);
+   if (e->arena) (void)hipFree(e->arena);
+   if (e->host) (void)hipHostFree(e->host);
+   if (e->stream) (void)hipStreamDestroy(e->stream);
+   delete e;
+}
+
+int lgpu_ensemble_upload(lgpu_ensemble *e, int32_t member, int field, const double *host)
+{
+   if (!e || !host || member < 0 || member >= e->nMembers) return LGPU_EINVAL;
+   bool isElem, isGrad;
+   double *base = field_base(e, field, isElem, isGrad);
+   if (!base) return LGPU_EINVAL;
+   const size_t n = isElem ? (size_t)e->elemCount[member] : (size_t)e->nodeCount[member];
+   const size_t off = isElem ? (size_t)e->elemStart[member] : (size_t)e->nodeStart[member];
+   ENS_HIP(e, hipStreamSynchronize(e->stream));
+   if (isGrad && e->precision == LGPU_MIXED) {
+      std::vector<float> tmp(n);
+      for (size_t i = 0; i < n; ++i) tmp[i] = (float)host[i];
+      ENS_HIP(e, hipMemcpy(reinterpret_cast<float *>(base) + off, tmp.data(), 4 * n, hipMemcpyHostToDevice));
+   } else {
+      ENS_HIP(e, hipMemcpy(base + off, host, 8 * n, hipMemcpyHostToDevice));
+   }
+   return LGPU_OK;
+}
+
+int lgpu_ensemble_download(lgpu_ensemble *e, int32_t member, int field, double *host)
+{
+   if (!e || !host || member < 0 || member >= e->nMembers) return LGPU_EINVAL;
+   bool isElem, isGrad;
+   double *base = field_base(e, field, isElem, isGrad);
+   if (!base) return LGPU_EINVAL;
+   const size_t n = isElem ? (size_t)e->elemCount[member] : (size_t)e->nodeCount[member];
+   const size_t off = isElem ? (size_t)e->elemStart[member] : (size_t)e->nodeStart[member];
+   ENS_HIP(e, hipStreamSynchronize(e->stream));
+   if (isGrad && e->precision == LGPU_MIXED) {
+      std::vector<float> tmp(n);
+      ENS_HIP(e, hipMemcpy(tmp.data(), reinterpret_cast<float *>(base) + off, 4 * n, hipMemcpyDeviceToHost));
+      for (size_t i = 0; i < n; ++i) host[i] = tmp[i];
+   } else {
+      ENS_HIP(e, hipMemcpy(host, base + off, 8 * n, hipMemcpyDeviceToHost));
+   }
+   return LGPU_OK;
+}
+
+int lgpu_ensemble_run(lgpu_ensemble *e, lgpu_time_state *ts, int32_t maxCycle, int32_t *status)
+{
+   if (!e || !ts) return LGPU_EINVAL;
+   const int M = e->nMembers;
+   // Pinned staging: 9 double arrays, cycle, active, err.
+   double *hd = reinterpret_cast<double *>(e->host);
+   int32_t *hcyc = reinterpret_cast<int32_t *>(hd + 9 * (size_t)M);
+   int32_t *hact = hcyc + M;
+   int32_t *herr = hact + M;
+   double *cols[9];
+   for (int i = 0; i < 9; ++i) cols[i] = hd + (size_t)i * M;
+   for (int m = 0; m < M; ++m) {
+      cols[0][m] = ts[m].time;
+      cols[1][m] = ts[m].deltatime;
+      cols[2][m] = ts[m].dtcourant;
+      cols[3][m] = ts[m].dthydro;
+      cols[4][m] = ts[m].stoptime;
+      cols[5][m] = ts[m].dtfixed;
+      cols[6][m] = ts[m].dtmax;
+      cols[7][m] = ts[m].deltatimemultlb;
+      cols[8][m] = ts[m].deltatimemultub;
+      hcyc[m] = ts[m].cycle;
+      hact[m] = 1;
+   }
+   EnsTime &t = e->t;
+   double *dcols[9] = {t.time, t.deltatime, t.dtcourant, t.dthydro, t.stoptime, t.dtfixed,
+                       t.dtmax, t.multlb, t.multub};
+   for (int i = 0; i < 9; ++i)
+      ENS_HIP(e, hipMemcpyAsync(dcols[i], cols[i], 8 * (size_t)M, hipMemcpyHostToDevice, e->stream));
+   ENS_HIP(e, hipMemcpyAsync(t.cycle, hcyc, 4 * (size_t)M, hipMemcpyHostToDevice, e->stream));
+   ENS_HIP(e, hipMemcpyAsync(t.active, hact, 4 * (size_t)M, hipMemcpyHostToDevice, e->stream));
+   // dt results feed the first TimeIncrement of members past cycle 0.
+   {
+      std::vector<double> dtout(2 * (size_t)M);
+      for (int m = 0; m < M; ++m) {
+         dtout[2 * (size_t)m] = ts[m].dtcourant;
+         dtout[2 * (size_t)m + 1] = ts[m].dthydro;
+      }
+      ENS_HIP(e, hipMemcpyAsync(e->s.memberDtOut, dtout.data(), 16 * (size_t)M,
+                                hipMemcpyHostToDevice, e->stream));
+      ENS_HIP(e, hipStreamSynchronize(e->stream));
+   }
+   if (e->useGraph && !e->timing && e->graphMaxCycle != maxCycle) {
+      const int r = build_graph(e, maxCycle);
+      if (r) return r;
+   }
+   const int batch = 8;
+   for (;;) {
+      for (int b = 0; b < batch; ++b) {
+         if (e->timing) {
+            float ms[kENumKernels];
+            const int r = timed_cycle_any(e, maxCycle, ms);
+            if (r) return r;
+            for (int i = 0; i < kENumKernels; ++i) {
+               e->kLaunches[i] += 1;
+               e->kMs[i] += ms[i];
+            }
+         } else if (e->useGraph) {
+            ENS_HIP(e, hipGraphLaunch(e->exec, e->stream));
+         } else {
+            enqueue_cycle(e, maxCycle);
+            ENS_HIP(e, hipGetLastError());
+         }
+      }
+      ENS_HIP(e, hipMemcpyAsync(hact, t.active, 4 * (size_t)M, hipMemcpyDeviceToHost, e->stream));
+      ENS_HIP(e, hipStreamSynchronize(e->stream));
+      bool any = false;
+      for (int m = 0; m < M; ++m) any = any || hact[m] != 0;
+      if (!any) break;
+   }
+   // Read back the time state; a member's last dt results are in
+   // memberDtOut (inactive members leave them untouched).
+   for (int i = 0; i < 9; ++i)
+      ENS_HIP(e, hipMemcpyAsync(cols[i], dcols[i], 8 * (size_t)M, hipMemcpyDeviceToHost, e->stream));
+   ENS_HIP(e, hipMemcpyAsync(hcyc, t.cycle, 4 * (size_t)M, hipMemcpyDeviceToHost, e->stream));
+   ENS_HIP(e, hipMemcpyAsync(herr, e->s.memberErr, 4 * (size_t)M * kNumErrSites,
+                             hipMemcpyDeviceToHost, e->stream));
+   std::vector<double> dtout(2 * (size_t)M);
+   ENS_HIP(e, hipMemcpyAsync(dtout.data(), e->s.memberDtOut, 16 * (size_t)M, hipMemcpyDeviceToHost,
+                             e->stream));
+   ENS_HIP(e, hipStreamSynchronize(e->stream));
+   int worst = LGPU_OK;
+   for (int m = 0; m < M; ++m) {
+      ts[m].time = cols[0][m];
+      ts[m].deltatime = cols[1][m];
+      ts[m].cycle = hcyc[m];
+      if (hcyc[m] > 0) {
+         ts[m].dtcourant = dtout[2 * (size_t)m];
+         ts[m].dthydro = dtout[2 * (size_t)m + 1];
+      }
+      const int32_t *er = herr + (size_t)m * kNumErrSites;
+      int st = LGPU_OK;
+      if (er[kErrForceDeterm] || er[kErrHourglassV] || er[kErrKinVnew]) st = LGPU_VOLUME_ERROR;
+      else if (er[kErrQStop]) st = LGPU_QSTOP_ERROR;
+      else if (er[kErrMaterialV]) st = LGPU_VOLUME_ERROR;
+      if (status) status[m] = st;
+      if (st != LGPU_OK && worst == LGPU_OK) worst = st;
+   }
+   return worst;
+}
+
+int lgpu_ensemble_set_option(lgpu_ensemble *e, int option, int64_t value)
+{
+   if (!e) return LGPU_EINVAL;
+   switch (option) {
+   case LGPU_ENS_OPT_GRAPH:
+      e->useGraph = value != 0;
+      drop_graph(e);
+      return LGPU_OK;
+   case LGPU_ENS_OPT_TIMING:
+      e->timing = value != 0;
+      return LGPU_OK;
+   case LGPU_ENS_OPT_BLOCK_FORCE:
+   case LGPU_ENS_OPT_BLOCK_NODE:
+   case LGPU_ENS_OPT_BLOCK_KIN:
+      if (value != 64 && value != 128 && value != 256) return LGPU_EINVAL;
+      e->block[option - LGPU_ENS_OPT_BLOCK_FORCE] = (int)value;
+      drop_graph(e);
+      return LGPU_OK;
+   case LGPU_ENS_OPT_XCD_REMAP:
+      e->s.xcdRemap = value ? 1 : 0;
+      drop_graph(e);
+      return LGPU_OK;
+   default:
+      return LGPU_EINVAL;
+   }
+}
+
+int lgpu_ensemble_get_option(lgpu_ensemble *e, int option, int64_t *value)
+{
+   if (!e || !value) return LGPU_EINVAL;
+   switch (option) {
+   case LGPU_ENS_OPT_GRAPH: *value = e->useGraph; return LGPU_OK;
+   case LGPU_ENS_OPT_TIMING: *value = e->timing; return LGPU_OK;
+   case LGPU_ENS_OPT_BLOCK_FORCE:
+   case LGPU_ENS_OPT_BLOCK_NODE:
+   case LGPU_ENS_OPT_BLOCK_KIN: *value = e->block[option - LGPU_ENS_OPT_BLOCK_FORCE]; return LGPU_OK;
+   case LGPU_ENS_OPT_XCD_REMAP: *value = e->s.xcdRemap; return LGPU_OK;
+   default: return LGPU_EINVAL;
+   }
+}
+
+int lgpu_ensemble_autotune(lgpu_ensemble *e, const lgpu_time_state *ts, int32_t trialCycles)
+{
+   if (!e || !ts || trialCycles < 1) return LGPU_EINVAL;
+   // The trials advance the state: snapshot the state part of the arena and
+   // restore it after every trial.
+   char *snap = nullptr;
+   ENS_HIP(e, hipMalloc((void **)&snap, e->stateBytes));
+   ENS_HIP(e, hipMemcpyAsync(snap, e->arena, e->stateBytes, hipMemcpyDeviceToDevice, e->stream));
+   const int M = e->nMembers;
+   std::vector<lgpu_time_state> tsv(ts, ts + M);
+   const bool savedTiming = e->timing, savedGraph = e->useGraph;
+   e->timing = true;
+   auto trial = [&](float ms[kENumKernels]) -> int {
+      for (int i = 0; i < kENumKernels; ++i) ms[i] = 0.f;
+      for (int i = 0; i < kENumKernels; ++i) {
+         e->kLaunches[i] = 0;
+         e->kMs[i] = 0.0;
+      }
+      std::vector<lgpu_time_state> t2 = tsv;
+      const int r = lgpu_ensemble_run(e, t2.data(), t2[0].cycle + trialCycles, nullptr);
+      if (r <= -100) return r;
+      for (int i = 0; i < kENumKernels; ++i)
+         ms[i] = e->kLaunches[i] ? (float)(e->kMs[i] / e->kLaunches[i]) : 0.f;
+      ENS_HIP(e, hipMemcpyAsync(e->arena, snap, e->stateBytes, hipMemcpyDeviceToDevice, e->stream));
+      ENS_HIP(e, hipStreamSynchronize(e->stream));
+      return LGPU_OK;
+   };
+   // Each of the three launch-shape-sensitive kernels is tuned over the block
+   // sizes, then the XCD mapping over the tuned shapes.
+   const int cands[3] = {64, 128, 256};
+   const int kern[3] = {kEForce, kENode, kEKin};
+   int r = LGPU_OK;
+   for (int which = 0; which < 3 && !r; ++which) {
+      float best = 1e30f;
+      int bestB = e->block[which];
+      for (int b : cands) {
+         e->block[which] = b;
+         float ms[kENumKernels];
+         r = trial(ms);
+         if (r) break;
+         if (ms[kern[which]] < best) {
+            best = ms[kern[which]];
+            bestB = b;
+         }
+      }
+      e->block[which] = bestB;
+   }
+   if (!r) {
+      float tot[2];
+      for (int remap = 0; remap < 2 && !r; ++remap) {
+         e->s.xcdRemap = remap;
+         float ms[kENumKernels];
+         r = trial(ms);
+         tot[remap] = ms[kEForce] + ms[kENode] + ms[kEKin] + ms[kEEos];
+      }
+      if (!r) e->s.xcdRemap = (tot[1] <= tot[0]) ? 1 : 0;
+   }
+   e->timing = savedTiming;
+   e->useGraph = savedGraph;
+   for (int i = 0; i < kENumKernels; ++i) {
+      e->kLaunches[i] = 0;
+      e->kMs[i] = 0.0;
+   }
+   drop_graph(e);
+   (void)hipFree(snap);
+   return r;
+}
+
+int lgpu_ensemble_kernel_stats(lgpu_ensemble *e, int kernel, int64_t *launches, double *total_ms)
+{
+   if (!e || kernel < 0 || kernel >= kENumKernels) return LGPU_EINVAL;
+   if (launches) *launches = e->kLaunches[kernel];
+   if (total_ms) *total_ms = e->kMs[kernel];
+   return LGPU_OK;
+}
+
+const char *lgpu_ensemble_kernel_name(int kernel)
+{
+   return (kernel >= 0 && kernel < kENumKernels) ? kEnsKernelNames[kernel] : "";
+}
+
+int lgpu_ensemble_num_kernels(void) { return kENumKernels; }
+
+int64_t lgpu_ensemble_num_elem(lgpu_ensemble *e) { return e ? e->s.numElem : 0; }
+
+const char *lgpu_ensemble_last_error(lgpu_ensemble *e) { return e ? e->lastError.c_str() : ""; }
+
+} // extern "C"

@@ -95,6 +95,7 @@ struct DevState {
    // corner planes: [(component*2 + kind)*8 + corner][elem], kind 0 stress, 1 hourglass
    void *corner;
    // region order
+   int32_t numPerm;         // entries of perm (>= numElem; -1 marks padding)
    const int32_t *perm;     // region-major element order
    const int32_t *regStart; // numReg+1 offsets into perm
    const int32_t *regRep;   // EOS repeat count per region
@@ -106,6 +107,17 @@ struct DevState {
    const int32_t *active;   // device-side loop flag (null: always active)
    Brick brick;             // valid when isBrick
    int32_t isBrick;
+   // Ensemble of independent problems (nMembers > 0): members are
+   // concatenated; each has its own dt, loop flag and error flags.
+   int32_t nMembers;
+   const int32_t *memberNodeStart; // nMembers+1
+   const int32_t *memberElemStart; // nMembers+1
+   const double *memberDt;         // per-member deltatime
+   const int32_t *memberActive;    // per-member loop flag
+   int32_t *memberErr;             // nMembers x kNumErrSites
+   const int32_t *blockMember;     // k_eos block -> member (-1: padding)
+   const int32_t *memberBlockStart; // nMembers+1: k_eos blocks of each member
+   double *memberDtOut;            // 2 per member: dtcourant, dthydro
 };
 
 } // namespace lgpu

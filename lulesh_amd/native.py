@@ -113,6 +113,29 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lgpu_get_info.restype = ctypes.c_int
         lib.lgpu_last_error.argtypes = [vp]
         lib.lgpu_last_error.restype = ctypes.c_char_p
+        lib.lgpu_ensemble_run.argtypes = [vp, ctypes.POINTER(TimeState), i32, _i32p]
+        lib.lgpu_ensemble_run.restype = ctypes.c_int
+        lib.lgpu_ensemble_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int64]
+        lib.lgpu_ensemble_set_option.restype = ctypes.c_int
+        lib.lgpu_ensemble_get_option.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        lib.lgpu_ensemble_get_option.restype = ctypes.c_int
+        lib.lgpu_ensemble_autotune.argtypes = [vp, ctypes.POINTER(TimeState), i32]
+        lib.lgpu_ensemble_autotune.restype = ctypes.c_int
+        lib.lgpu_ensemble_num_kernels.argtypes = []
+        lib.lgpu_ensemble_num_kernels.restype = ctypes.c_int
+        lib.lgpu_ensemble_kernel_name.argtypes = [ctypes.c_int]
+        lib.lgpu_ensemble_kernel_name.restype = ctypes.c_char_p
+        lib.lgpu_ensemble_kernel_stats.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                                   _f64p]
+        lib.lgpu_ensemble_kernel_stats.restype = ctypes.c_int
+        lib.lgpu_ensemble_num_elem.argtypes = [vp]
+        lib.lgpu_ensemble_num_elem.restype = ctypes.c_int64
+        lib.lgpu_ensemble_last_error.argtypes = [vp]
+        lib.lgpu_ensemble_last_error.restype = ctypes.c_char_p
+        lib.lgpu_ensemble_download.argtypes = [vp, i32, ctypes.c_int, _f64p]
+        lib.lgpu_ensemble_download.restype = ctypes.c_int
+        lib.lgpu_ensemble_upload.argtypes = [vp, i32, ctypes.c_int, _f64p]
+        lib.lgpu_ensemble_upload.restype = ctypes.c_int
         _GPU = lib
     return _GPU
 
@@ -154,6 +177,22 @@ def host_lib() -> ctypes.CDLL:
         lib.lh_symmetry.restype = None
         lib.lh_glibc_rand.argtypes = [ctypes.c_uint32, i32, _i32p]
         lib.lh_glibc_rand.restype = None
+        lib.lh_ensemble_draw.argtypes = [ctypes.c_uint64, i32, _i32p, _f64p, _i32p, _i32p]
+        lib.lh_ensemble_draw.restype = None
+        lib.lh_ensemble_create.argtypes = [i32, _i32p, _f64p, _i32p, _i32p, _i32p, i32]
+        lib.lh_ensemble_create.restype = vp
+        lib.lh_ensemble_destroy.argtypes = [vp]
+        lib.lh_ensemble_destroy.restype = None
+        lib.lh_ensemble_run.argtypes = [vp, i32, _i32p]
+        lib.lh_ensemble_run.restype = ctypes.c_int
+        lib.lh_ensemble_sync.argtypes = [vp]
+        lib.lh_ensemble_sync.restype = ctypes.c_int
+        lib.lh_ensemble_member.argtypes = [vp, i32]
+        lib.lh_ensemble_member.restype = vp
+        lib.lh_ensemble_device.argtypes = [vp]
+        lib.lh_ensemble_device.restype = vp
+        lib.lh_ensemble_size.argtypes = [vp]
+        lib.lh_ensemble_size.restype = i32
         _HOST = lib
     return _HOST
 
@@ -284,19 +323,22 @@ class HostDomain:
     """The C++ host Domain (lulesh_amd/csrc/host/domain.h) through its C API."""
 
     def __init__(self, s: int, nr: int = 11, balance: int = 1, cost: int = 1,
-                 emult: float = 1.0):
+                 emult: float = 1.0, _view=None, _owner=None):
         self._lib = host_lib()
-        self.ptr = self._lib.lh_create(s, nr, balance, cost, float(emult))
+        self._owned = _view is None
+        self._owner = _owner  # keeps an ensemble alive while its member is viewed
+        self.ptr = self._lib.lh_create(s, nr, balance, cost, float(emult)) if _view is None else _view
         if not self.ptr:
             raise ValueError("lh_create failed")
-        self.s = s
         self.numElem = self._lib.lh_num_elem(self.ptr)
         self.numNode = self._lib.lh_num_node(self.ptr)
         self.numReg = self._lib.lh_num_reg(self.ptr)
+        self.s = round(self.numElem ** (1 / 3))
 
     def close(self):
         if getattr(self, "ptr", None):
-            self._lib.lh_destroy(self.ptr)
+            if self._owned:
+                self._lib.lh_destroy(self.ptr)
             self.ptr = None
 
     def __del__(self):
@@ -338,3 +380,96 @@ class HostDomain:
         out = (ctypes.c_double * 3)()
         self._lib.lh_symmetry(self.ptr, out)
         return out[0], out[1], out[2]
+
+
+def ensemble_draw(n: int, seed: int = 1234):
+    """Member parameters of lh_ensemble_draw: (s, emult, balance, cost) arrays."""
+    sz = np.zeros(n, np.int32)
+    em = np.zeros(n, np.float64)
+    b = np.zeros(n, np.int32)
+    c = np.zeros(n, np.int32)
+    host_lib().lh_ensemble_draw(seed, n, sz.ctypes.data_as(_i32p), em.ctypes.data_as(_f64p),
+                                b.ctypes.data_as(_i32p), c.ctypes.data_as(_i32p))
+    return sz, em, b, c
+
+
+ENS_OPT_GRAPH, ENS_OPT_TIMING, ENS_OPT_BLOCK_FORCE, ENS_OPT_BLOCK_NODE, ENS_OPT_BLOCK_KIN, \
+    ENS_OPT_XCD_REMAP = 1, 2, 3, 4, 5, 6
+
+
+class Ensemble:
+    """Host Domains advanced together on the device (lh_ensemble / lgpu_ensemble)."""
+
+    def __init__(self, sizes, emults, balances, costs, regions=11, precision: int = FP64):
+        lib = host_lib()
+        n = len(sizes)
+        self._arrays = [np.ascontiguousarray(sizes, np.int32),
+                        np.ascontiguousarray(emults, np.float64),
+                        np.full(n, regions, np.int32) if np.isscalar(regions)
+                        else np.ascontiguousarray(regions, np.int32),
+                        np.ascontiguousarray(balances, np.int32),
+                        np.ascontiguousarray(costs, np.int32)]
+        sz, em, nr, b, c = self._arrays
+        self.ptr = lib.lh_ensemble_create(n, sz.ctypes.data_as(_i32p), em.ctypes.data_as(_f64p),
+                                          nr.ctypes.data_as(_i32p), b.ctypes.data_as(_i32p),
+                                          c.ctypes.data_as(_i32p), precision)
+        if not self.ptr:
+            raise RuntimeError("lh_ensemble_create failed")
+        self.n = n
+        self.dev = lib.lh_ensemble_device(self.ptr)
+        self.members = [HostDomain(0, _view=lib.lh_ensemble_member(self.ptr, i), _owner=self)
+                        for i in range(n)]
+        self.num_elem = int(gpu_lib().lgpu_ensemble_num_elem(self.dev))
+
+    def close(self):
+        if getattr(self, "ptr", None):
+            for m in self.members:
+                m.ptr = None
+            host_lib().lh_ensemble_destroy(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        self.close()
+
+    def run(self, its: int):
+        status = np.zeros(self.n, np.int32)
+        st = check(host_lib().lh_ensemble_run(self.ptr, its, status.ctypes.data_as(_i32p)))
+        return st, status
+
+    def sync(self) -> None:
+        check(host_lib().lh_ensemble_sync(self.ptr))
+
+    def set_option(self, option: int, value: int) -> None:
+        check(gpu_lib().lgpu_ensemble_set_option(self.dev, option, value))
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int64()
+        check(gpu_lib().lgpu_ensemble_get_option(self.dev, option, ctypes.byref(v)))
+        return v.value
+
+    def time_states(self):
+        arr = (TimeState * self.n)()
+        for i, m in enumerate(self.members):
+            t = arr[i]
+            for name in ("time", "deltatime", "dtcourant", "dthydro", "stoptime", "dtfixed",
+                         "dtmax", "deltatimemultlb", "deltatimemultub"):
+                setattr(t, name, m.get(name))
+            t.cycle = int(m.get("cycle"))
+        return arr
+
+    def autotune(self, trial_cycles: int = 3) -> dict:
+        ts = self.time_states()
+        check(gpu_lib().lgpu_ensemble_autotune(self.dev, ts, trial_cycles))
+        return {"block_force": self.get_option(ENS_OPT_BLOCK_FORCE),
+                "block_node": self.get_option(ENS_OPT_BLOCK_NODE),
+                "block_kin": self.get_option(ENS_OPT_BLOCK_KIN),
+                "xcd_remap": self.get_option(ENS_OPT_XCD_REMAP)}
+
+    def kernel_stats(self) -> dict:
+        lib = gpu_lib()
+        out = {}
+        for k in range(lib.lgpu_ensemble_num_kernels()):
+            n, ms = ctypes.c_int64(), ctypes.c_double()
+            check(lib.lgpu_ensemble_kernel_stats(self.dev, k, ctypes.byref(n), ctypes.byref(ms)))
+            out[lib.lgpu_ensemble_kernel_name(k).decode()] = (n.value, ms.value)
+        return out

@@ -1,0 +1,103 @@
+"""The batched ensemble: independent problems of different sizes, each with
+its own dt, advanced by one captured cycle graph.
+
+FP64 members must equal single-problem runs of the same member bit for bit
+(both follow the reference's operation order), end where their own loop test
+ends them, and reproduce the reference's end-of-run numbers.  Mixed members
+stay within the mixed form's bound of the FP64 reference.
+"""
+import numpy as np
+import pytest
+
+import lulesh_amd as la
+from helpers import STATE_FIELDS, max_rel, summary
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(sizes=[4, 7, 5, 10, 6, 9, 3, 8], emults=[1.0, 0.7, 1.9, 1.25, 0.5, 1.0, 1.4, 0.9],
+             balances=[1, 2, 1, 2, 2, 1, 1, 2], costs=[1, 1, 2, 2, 1, 2, 1, 1])
+
+
+def _single(s, b, c, m, its, precision=la.FP64):
+    h = la.HostDomain(s, 11, b, c, m)
+    h.attach_device(precision)
+    assert h.run(its) == 0
+    h.sync()
+    return h
+
+
+@pytest.mark.parametrize("graph", [True, False])
+def test_members_equal_single_runs(graph):
+    ens = la.Ensemble(**SMALL)
+    ens.set_option(la.native.ENS_OPT_GRAPH, int(graph))
+    st, status = ens.run(40)
+    assert st == 0 and not status.any()
+    ens.sync()
+    for i, mem in enumerate(ens.members):
+        s, m = SMALL["sizes"][i], SMALL["emults"][i]
+        b, c = SMALL["balances"][i], SMALL["costs"][i]
+        ref = _single(s, b, c, m, 40)
+        assert mem.get("cycle") == ref.get("cycle")
+        assert mem.get("time") == ref.get("time")
+        assert mem.get("deltatime") == ref.get("deltatime")
+        for f in STATE_FIELDS:
+            assert np.array_equal(mem.real(f), ref.real(f)), (i, f)
+
+
+def test_members_stop_at_their_own_end():
+    """s=2,3,4 reach t=1e-2 after 5, 23, 45 cycles (reference); an s=6 member
+    runs on to the cycle limit."""
+    ens = la.Ensemble(sizes=[2, 3, 4, 6], emults=[1.0] * 4, balances=[1] * 4, costs=[1] * 4)
+    st, _ = ens.run(60)
+    assert st == 0
+    ens.sync()
+    for mem, s in zip(ens.members[:3], (2, 3, 4)):
+        ref = summary(s, 11, 1, 1, 9999999)
+        assert int(mem.get("cycle")) == ref["cycles"]
+        assert abs(mem.real("e")[0] - float(ref["e0"])) <= 1e-10 * abs(float(ref["e0"]))
+    assert int(ens.members[3].get("cycle")) == 60
+
+
+def test_members_match_reference_numbers():
+    keys = [(16, 11, 2, 2, 100, 0.5), (20, 11, 1, 2, 100, 1.7), (24, 11, 2, 1, 100, 1.25),
+            (32, 11, 1, 1, 100, 0.8), (48, 11, 2, 2, 100, 2.0)]
+    ens = la.Ensemble(sizes=[k[0] for k in keys], emults=[k[5] for k in keys],
+                      balances=[k[2] for k in keys], costs=[k[3] for k in keys])
+    st, _ = ens.run(100)
+    assert st == 0
+    ens.sync()
+    for mem, k in zip(ens.members, keys):
+        ref = summary(*k)
+        assert int(mem.get("cycle")) == ref["cycles"] == 100
+        e0 = mem.real("e")[0]
+        assert abs(e0 - float(ref["e0"])) <= 1e-10 * float(ref["e0"]), (k, e0, ref["e0"])
+
+
+def test_mixed_ensemble_within_bound():
+    keys = [(16, 11, 2, 2, 100, 0.5), (24, 11, 2, 1, 100, 1.25), (48, 11, 2, 2, 100, 2.0)]
+    ens = la.Ensemble(sizes=[k[0] for k in keys], emults=[k[5] for k in keys],
+                      balances=[k[2] for k in keys], costs=[k[3] for k in keys],
+                      precision=la.MIXED)
+    st, _ = ens.run(100)
+    assert st == 0
+    ens.sync()
+    for mem, k in zip(ens.members, keys):
+        ref = summary(*k)
+        err = abs(mem.real("e")[0] - float(ref["e0"])) / float(ref["e0"])
+        print(f"mixed ensemble member {k}: e0 rel err {err:.2e}")
+        assert err <= 1e-6
+
+
+def test_autotune_keeps_state():
+    ens = la.Ensemble(**SMALL)
+    before = [m.get("cycle") for m in ens.members]
+    choice = ens.autotune(trial_cycles=2)
+    assert choice["block_force"] in (64, 128, 256)
+    st, _ = ens.run(10)
+    assert st == 0
+    ens.sync()
+    ref = _single(SMALL["sizes"][3], SMALL["balances"][3], SMALL["costs"][3], SMALL["emults"][3], 10)
+    mem = ens.members[3]
+    assert before[3] == 0 and mem.get("cycle") == 10
+    for f in STATE_FIELDS:
+        assert np.array_equal(mem.real(f), ref.real(f)), f
