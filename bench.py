@@ -7,16 +7,23 @@ the load-imbalanced s=192 FP64 configuration the headline FOM is quoted on.
 Prints one JSON line.
 
   value      FOM = numElem * steps / elapsed_us (zones/us), state resident in
-             HBM, per-cycle dt read back to the host as the reference's loop
-             needs (--device-loop keeps TimeIncrement on the device)
+             HBM, TimeIncrement evaluated on the device (lgpu_run); the
+             reference-style loop with a per-cycle dt read-back to the host is
+             reported beside it as host_loop_zones_per_us (--host-loop makes it
+             the headline)
   roofline   dominant kernel: algorithmic bytes per launch / mean launch time
              (HIP events on the context stream, measured in the timed region)
   cpu_baseline  the reference's own OpenMP build (oracle/_ref/lulesh2.0) on
              this host's cores, on a bounded sample of the same workload
+  mixed      mixed-precision form vs FP64 at s=192, 11 regions, 200 cycles,
+             with the origin-energy error against the reference's number
+  ensemble   64 Sedov problems (std::mt19937_64(1234) draw), 100 cycles,
+             batched vs run one after another
 """
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import re
@@ -24,6 +31,8 @@ import subprocess
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -82,6 +91,128 @@ def cpu_baseline(s: int, r: int, b: int, c: int, seconds: float) -> dict | None:
                       f"(OpenMP {cores} threads, cycle loop only)"}
 
 
+def golden_e0(s: int, r: int, b: int, c: int, its: int) -> float | None:
+    """Final origin energy of the reference build for this configuration, if
+    recorded in tests/golden/ref_summaries.json."""
+    p = ROOT / "tests" / "golden" / "ref_summaries.json"
+    try:
+        d = json.loads(p.read_text())
+        return float(d[f"s{s}_r{r}_b{b}_c{c}_i{its}_m1"]["e0"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def timed_run(la, torch, s, r, b, c, prec, warmup, steps, device_loop, timing=False):
+    """Builds the problem, runs warm-up cycles, times `steps` cycles.
+    Returns (zones_per_us, ms_per_step, domain, (warm-up per-kernel stats,
+    timed-region stats of the dominant kernel))."""
+    dom = la.HostDomain(s, r, b, c)
+    dom.attach_device(prec)
+    ctx = dom.device()
+    lib = la.gpu_lib()
+
+    def stats():
+        out = {}
+        for k in range(lib.lgpu_num_kernels()):
+            n, ms = ctypes.c_int64(), ctypes.c_double()
+            la.check(lib.lgpu_kernel_stats(ctx, k, ctypes.byref(n), ctypes.byref(ms)))
+            if n.value:
+                out[lib.lgpu_kernel_name(k).decode()] = (n.value, ms.value)
+        return out
+
+    if timing:
+        # Warm-up with every kernel timed: per-kernel breakdown and the
+        # dominant kernel; the timed region then times only that one.
+        la.check(lib.lgpu_set_option(ctx, 2, 1))
+    if dom.run(warmup, device_loop) != 0:
+        raise RuntimeError("warm-up failed")
+    breakdown = {}
+    if timing:
+        breakdown = stats()
+        dom_k = max(breakdown, key=lambda k: breakdown[k][1])
+        kid = [lib.lgpu_kernel_name(k).decode() for k in range(lib.lgpu_num_kernels())].index(dom_k)
+        la.check(lib.lgpu_set_option(ctx, 6, 1 << kid))
+        la.check(lib.lgpu_reset_kernel_stats(ctx))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st = dom.run(warmup + steps, device_loop)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    if st != 0:
+        raise RuntimeError(f"timed run failed with status {st}")
+    done = int(dom.get("cycle")) - warmup
+    timed_k = stats() if timing else {}
+    return s ** 3 * done / ((t1 - t0) * 1e6), (t1 - t0) * 1e3 / done, dom, (breakdown, timed_k)
+
+
+def mixed_leg(la, torch, s: int, steps: int) -> dict:
+    """Mixed vs FP64 on the s, 11-region, b=1, c=1 problem, both run from cycle
+    0 for the same number of cycles; energy error against the reference's
+    FP64 number where it is recorded."""
+    warm = 5
+    total = warm + steps
+    fm, msm, dm, _ = timed_run(la, torch, s, 11, 1, 1, la.MIXED, warm, steps, True)
+    fd, msd, dd, _ = timed_run(la, torch, s, 11, 1, 1, la.FP64, warm, steps, True)
+    dm.sync()
+    dd.sync()
+    e_m, e_d = float(dm.real("e")[0]), float(dd.real("e")[0])
+    ref = golden_e0(s, 11, 1, 1, total)
+    out = {"workload": f"sedov s={s} r=11 b=1 c=1, cycles {total}",
+           "mixed_zones_per_us": round(fm, 2), "fp64_zones_per_us": round(fd, 2),
+           "speedup": round(fm / fd, 3), "cycles": total,
+           "e0_mixed": e_m, "e0_fp64": e_d,
+           "e0_rel_err_vs_fp64_gpu": abs(e_m - e_d) / abs(e_d)}
+    if ref is not None:
+        out["e0_reference"] = ref
+        out["e0_rel_err_vs_reference"] = abs(e_m - ref) / abs(ref)
+        out["fp64_e0_rel_err_vs_reference"] = abs(e_d - ref) / abs(ref)
+    for f in ("e", "p", "q", "v", "xd"):
+        a, b_ = dm.real(f), dd.real(f)
+        out[f"linf_{f}"] = float(abs(a - b_).max() / max(abs(b_).max(), 1e-300))
+    return out
+
+
+def ensemble_leg(la, torch, members: int, cycles: int, seed: int = 1234) -> dict:
+    """The batched ensemble (one cycle graph per cycle for all members) against
+    running the same members one after another (each on its own context,
+    device-side time stepping)."""
+    sz, em, b, c = la.ensemble_draw(members, seed)
+    zones = int((sz.astype(np.int64) ** 3).sum()) * cycles
+    out = {"members": members, "seed": seed, "cycles": cycles,
+           "total_elements": int((sz.astype(np.int64) ** 3).sum())}
+    for name, prec in (("fp64", la.FP64), ("mixed", la.MIXED)):
+        ens = la.Ensemble(sz, em, b, c, precision=prec)
+        tune = ens.autotune(trial_cycles=3)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        st, status = ens.run(cycles)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        if st != 0:
+            raise RuntimeError(f"ensemble failed: {status}")
+        out[f"{name}_batched_zones_per_us"] = round(zones / ((t1 - t0) * 1e6), 2)
+        out[f"{name}_autotune"] = tune
+        ens.close()
+    singles = []
+    for i in range(members):
+        d = la.HostDomain(int(sz[i]), 11, int(b[i]), int(c[i]), float(em[i]))
+        d.attach_device(la.FP64)
+        singles.append(d)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for d in singles:
+        if d.run(cycles, True) != 0:
+            raise RuntimeError("sequential member failed")
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    out["fp64_sequential_zones_per_us"] = round(zones / ((t1 - t0) * 1e6), 2)
+    out["fp64_speedup_vs_sequential"] = round(out["fp64_batched_zones_per_us"] /
+                                              out["fp64_sequential_zones_per_us"], 3)
+    for d in singles:
+        d.close()
+    return out
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -92,11 +223,20 @@ def main() -> None:
     ap.add_argument("--balance", type=int, default=2)
     ap.add_argument("--cost", type=int, default=2)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
-    ap.add_argument("--device-loop", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
+    ap.add_argument("--host-loop", action="store_true",
+                    help="TimeIncrement on the host with a per-cycle dt read-back "
+                         "(default: on the device)")
+    ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-members", type=int, default=64)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="first run one device copy of 512 MiB (float64), the known-size "
+                         "kernel that scales the FETCH_SIZE/WRITE_SIZE counters")
     a = ap.parse_args()
+    a.device_loop = not a.host_loop
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {a.gpus} is not supported "
               "(the multi-GPU halo exchange is design-only)", file=sys.stderr)
@@ -108,41 +248,25 @@ def main() -> None:
 
     if not torch.cuda.is_available():
         sys.exit("bench.py: no GPU visible")
+    if a.calibrate_copy:
+        src = torch.ones(64 * 1024 * 1024, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     prec = la.MIXED if a.precision == "mixed" else la.FP64
-    print(f"[bench] building s={a.size} r={a.regions} b={a.balance} c={a.cost} {a.precision}",
+    print(f"[bench] s={a.size} r={a.regions} b={a.balance} c={a.cost} {a.precision}",
           file=sys.stderr, flush=True)
-    dom = la.HostDomain(a.size, a.regions, a.balance, a.cost)
-    dom.attach_device(prec)
-    ctx = dom.device()
-    lib = la.gpu_lib()
-
-    st = dom.run(a.warmup, a.device_loop)
-    if st != 0:
-        sys.exit(f"warm-up failed with status {st}")
-    if not a.no_kernel_timing:
-        la.check(lib.lgpu_set_option(ctx, 2, 1))
-        la.check(lib.lgpu_reset_kernel_stats(ctx))
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    st = dom.run(a.warmup + a.steps, a.device_loop)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    if st != 0:
-        sys.exit(f"timed run failed with status {st}")
+    fom, ms_step, dom, (kbreak, kstats) = timed_run(la, torch, a.size, a.regions, a.balance,
+                                                    a.cost, prec, a.warmup, a.steps,
+                                                    a.device_loop, timing=not a.no_kernel_timing)
     steps = int(dom.get("cycle")) - a.warmup
-    elapsed = t1 - t0
     E = a.size ** 3
-    fom = E * steps / (elapsed * 1e6)
+    del dom
 
     roof = None
-    kstats = {}
-    if not a.no_kernel_timing:
-        import ctypes
-        for k in range(lib.lgpu_num_kernels()):
-            n, ms = ctypes.c_int64(), ctypes.c_double()
-            la.check(lib.lgpu_kernel_stats(ctx, k, ctypes.byref(n), ctypes.byref(ms)))
-            if n.value:
-                kstats[lib.lgpu_kernel_name(k).decode()] = (n.value, ms.value)
+    if kstats:
+        # Dominant kernel: largest total time in the timed region.
         dom_k = max(kstats, key=lambda k: kstats[k][1])
         n, ms = kstats[dom_k]
         avg_s = ms / n / 1e3
@@ -151,6 +275,19 @@ def main() -> None:
         roof = {"bound": "hbm", "kernel": dom_k, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
                 "avg_us": round(avg_s * 1e6, 2), "alg_bytes": int(byts)}
+
+    extra = {}
+    print("[bench] host-loop variant", file=sys.stderr, flush=True)
+    fom_h, ms_h, dom_h, _ = timed_run(la, torch, a.size, a.regions, a.balance, a.cost, prec,
+                                      a.warmup, min(a.steps, 100), False)
+    del dom_h
+    extra["host_loop_zones_per_us"] = round(fom_h, 2)
+    if not a.no_mixed:
+        print("[bench] mixed-precision leg", file=sys.stderr, flush=True)
+        extra["mixed"] = mixed_leg(la, torch, a.size, 195)
+    if not a.no_ensemble:
+        print("[bench] ensemble leg", file=sys.stderr, flush=True)
+        extra["ensemble"] = ensemble_leg(la, torch, a.ensemble_members, 100)
 
     cpu = None
     if not a.no_cpu_baseline:
@@ -165,7 +302,7 @@ def main() -> None:
         "n_gpus": 1,
         "steps": steps,
         "warmup": a.warmup,
-        "ms_per_step": round(elapsed * 1e3 / steps, 4),
+        "ms_per_step": round(ms_step, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -178,7 +315,8 @@ def main() -> None:
                    "parallelism": "single GPU"},
         "roofline": roof,
         "cpu_baseline": cpu,
-        "kernels_ms_per_step": {k: round(v[1] / v[0], 4) for k, v in kstats.items()},
+        "kernels_ms_per_step": {k: round(v[1] / v[0], 4) for k, v in kbreak.items()},
+        **extra,
     }
     print(json.dumps(line), flush=True)
 

@@ -158,8 +158,10 @@ enum lgpu_option {
                                      or 256                                  */
    LGPU_OPT_BRICK_CHUNK = 4,      /* node planes per workgroup of the fused
                                      brick kernel (1..mesh edge)             */
-   LGPU_OPT_XCD_REMAP = 5         /* 1 (default): contiguous block ranges per
+   LGPU_OPT_XCD_REMAP = 5,        /* 1 (default): contiguous block ranges per
                                      XCD; 0: hardware order                 */
+   LGPU_OPT_KERNEL_TIMING_MASK = 6 /* bit k: time kernel k while timing is on
+                                     (default: all)                          */
 };
 int lgpu_set_option(lgpu_ctx *ctx, int option, int64_t value);
 

@@ -25,6 +25,9 @@ void lh_destroy(lh_domain *d);
 
 /* Creates the device context for this domain and uploads its state. */
 int lh_attach_device(lh_domain *d, int32_t precision);
+
+/* As lh_attach_device with lgpu_mesh_desc.flags (e.g. LGPU_FLAG_GENERIC). */
+int lh_attach_device_flags(lh_domain *d, int32_t precision, int32_t flags);
 lgpu_ctx *lh_device(lh_domain *d);
 
 /* One iteration of the main loop: TimeIncrement + LagrangeLeapFrog

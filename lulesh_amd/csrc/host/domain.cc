@@ -282,7 +282,7 @@ const int kStateFields[] = {LGPU_X, LGPU_Y, LGPU_Z, LGPU_XD, LGPU_YD, LGPU_ZD, L
 const int kDerivedFields[] = {LGPU_VNEW, LGPU_DELV, LGPU_VDOV, LGPU_AREALG};
 } // namespace
 
-int Domain::AttachDevice(int precision)
+int Domain::AttachDevice(int precision, int flags)
 {
    if (m_gpu) {
       lgpu_destroy(m_gpu);
@@ -317,6 +317,7 @@ int Domain::AttachDevice(int precision)
    d.regElemlist = lists.data();
    d.cost = m_cost;
    d.precision = precision;
+   d.flags = flags;
    d.k = m_k;
    int st = lgpu_create(&d, &m_gpu);
    if (st) return st;
@@ -507,6 +508,11 @@ void lh_destroy(lh_domain *h)
 int lh_attach_device(lh_domain *h, int32_t precision)
 {
    return h ? h->d->AttachDevice(precision) : LGPU_EINVAL;
+}
+
+int lh_attach_device_flags(lh_domain *h, int32_t precision, int32_t flags)
+{
+   return h ? h->d->AttachDevice(precision, flags) : LGPU_EINVAL;
 }
 
 lgpu_ctx *lh_device(lh_domain *h) { return h ? h->d->device() : nullptr; }

@@ -37,7 +37,7 @@ class Domain {
    void ScaleInitialEnergy(Real_t m);
 
    // Device mirror.
-   int AttachDevice(int precision); // lgpu_create + upload of the state
+   int AttachDevice(int precision, int flags = 0); // lgpu_create + upload of the state
    lgpu_ctx *device() { return m_gpu; }
    int SyncFromDevice();            // download every state field
 

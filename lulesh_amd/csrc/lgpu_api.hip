@@ -55,6 +55,7 @@ struct lgpu_ctx {
    int block = 256;
    bool keepDiag = false;
    bool timing = false;
+   uint32_t timingMask = 0xffffffffu;
    Consts k{};
    char *arena = nullptr;
    size_t arenaBytes = 0;
@@ -121,14 +122,15 @@ template <class F>
 int timed(lgpu_ctx *c, int kernel, F &&launch)
 {
    TimedLaunch t{kernel, nullptr, nullptr};
-   if (c->timing) {
+   const bool on = c->timing && ((c->timingMask >> kernel) & 1u);
+   if (on) {
       t.start = take_event(c);
       t.stop = take_event(c);
       if (t.start && t.stop) HIP_OR_RETURN(c, hipEventRecord(t.start, c->stream));
    }
    launch();
    HIP_OR_RETURN(c, hipGetLastError());
-   if (c->timing && t.start && t.stop) {
+   if (on && t.start && t.stop) {
       HIP_OR_RETURN(c, hipEventRecord(t.stop, c->stream));
       c->pending.push_back(t);
    }
@@ -753,6 +755,9 @@ int lgpu_set_option(lgpu_ctx *c, int option, int64_t value)
       return LGPU_OK;
    case LGPU_OPT_XCD_REMAP:
       c->s.xcdRemap = value ? 1 : 0;
+      return LGPU_OK;
+   case LGPU_OPT_KERNEL_TIMING_MASK:
+      c->timingMask = (uint32_t)value;
       return LGPU_OK;
    default:
       return LGPU_EINVAL;

@@ -154,6 +154,8 @@ def host_lib() -> ctypes.CDLL:
         lib.lh_destroy.restype = None
         lib.lh_attach_device.argtypes = [vp, i32]
         lib.lh_attach_device.restype = ctypes.c_int
+        lib.lh_attach_device_flags.argtypes = [vp, i32, i32]
+        lib.lh_attach_device_flags.restype = ctypes.c_int
         lib.lh_device.argtypes = [vp]
         lib.lh_device.restype = vp
         lib.lh_step.argtypes = [vp]
@@ -361,8 +363,8 @@ class HostDomain:
     def set(self, name: str, v: float) -> None:
         self._lib.lh_set(self.ptr, SCALARS.index(name), float(v))
 
-    def attach_device(self, precision: int = FP64) -> None:
-        check(self._lib.lh_attach_device(self.ptr, precision))
+    def attach_device(self, precision: int = FP64, flags: int = 0) -> None:
+        check(self._lib.lh_attach_device_flags(self.ptr, precision, flags))
 
     def device(self):
         return self._lib.lh_device(self.ptr)

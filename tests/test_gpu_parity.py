@@ -148,12 +148,23 @@ def test_s30_to_stoptime_matches_reference(device_loop):
     print(f"s=30: cycles {int(h.get('cycle'))} e0 {e0:.17e} (ref {ref['e0']}) sym {sym}")
 
 
-def test_device_loop_equals_host_loop():
-    a = _run_host(12, 11, 1, 1, 60, device_loop=False)
-    b = _run_host(12, 11, 1, 1, 60, device_loop=True)
+@pytest.mark.parametrize("generic", [0, 1])
+def test_device_loop_equals_host_loop(generic):
+    """Several device-loop runs in a row whose ends fall inside a launch
+    batch (5, 13, 40, 60 cycles) give exactly the host loop's state."""
+    a = la.HostDomain(12, 11, 1, 1)
+    a.attach_device(la.FP64, generic)
+    assert a.run(60, False) == 0
+    a.sync()
+    b = la.HostDomain(12, 11, 1, 1)
+    b.attach_device(la.FP64, generic)
+    for its in (5, 13, 40, 60):
+        assert b.run(its, True) == 0
+    b.sync()
     assert a.get("cycle") == b.get("cycle") == 60
     assert a.get("time") == b.get("time")
     assert a.get("deltatime") == b.get("deltatime")
+    assert a.get("dtcourant") == b.get("dtcourant")
     for f in STATE_FIELDS:
         assert np.array_equal(a.real(f), b.real(f)), f
 
