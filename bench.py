@@ -40,7 +40,14 @@ sys.path.insert(0, str(ROOT))
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 
 
-def kernel_bytes(name: str, s: int, precision: str) -> float:
+FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector
+# Algorithmic FP64 operations per element of the fused nodal pass (one
+# element's stress integration 376, hourglass control 1379, node update 66;
+# counted from the reference formulas).
+NODAL_FLOPS_PER_ELEM = 1821
+
+
+def kernel_bytes(name: str, s: int, precision: str, brick: bool = True) -> float:
     """Algorithmic (compulsory) HBM bytes of one launch of each kernel: every
     array the kernel must read or write counted once, node gathers assumed
     perfectly cached (r = nodes per element)."""
@@ -48,6 +55,7 @@ def kernel_bytes(name: str, s: int, precision: str) -> float:
     r = ((s + 1) / s) ** 3
     cb = 4 if precision == "mixed" else 8   # corner force storage
     gb = 4 if precision == "mixed" else 8   # gradient storage
+    nl = 0 if brick else 32                 # element-to-node map (brick: computed)
     per_elem = {
         # nodelist 32, x..zd 48r, p q v volo ss elemMass 48 -> 48 corner values
         "k_force": 32 + 48 * r + 48 + 48 * cb,
@@ -57,7 +65,7 @@ def kernel_bytes(name: str, s: int, precision: str) -> float:
         # nodalMass 8r, node BC mask 1r, x..zd written 48r
         "k_nodal_brick": 48 * r + 48 + 8 * r + 1 * r + 48 * r,
         # nodelist 32, x..zd 48r, volo v 16 -> vnew delv arealg vdov 32, grads 6
-        "k_kin": 32 + 48 * r + 16 + 32 + 6 * gb,
+        "k_kin": nl + 48 * r + 16 + 32 + 6 * gb,
         # perm 4, elemBC 4, 6 nbr 24, grads 6, 10 fields 80 -> 5 fields 40
         "k_eos": 4 + 4 + 24 + 6 * gb + 80 + 40,
     }
@@ -89,6 +97,19 @@ def cpu_baseline(s: int, r: int, b: int, c: int, seconds: float) -> dict | None:
             "kind": "reference",
             "sample": f"oracle/_ref/lulesh2.0 -s {s} -r {r} -b {b} -c {c} -i {its} "
                       f"(OpenMP {cores} threads, cycle loop only)"}
+
+
+def measured_traffic(kernel: str, precision: str, s: int):
+    """HBM bytes per launch of `kernel` from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs recorded in profiles/traffic.json (s=192 only), or None."""
+    try:
+        d = json.loads((ROOT / "profiles" / "traffic.json").read_text())
+        k = d[precision][kernel]
+    except (OSError, KeyError, ValueError):
+        return None
+    if s != 192:
+        return None
+    return int(1024 * (d["fetch_scale"] * k["fetch_kb"] + d["write_scale"] * k["write_kb"]))
 
 
 def golden_e0(s: int, r: int, b: int, c: int, its: int) -> float | None:
@@ -273,8 +294,14 @@ def main() -> None:
         byts = kernel_bytes(dom_k, a.size, a.precision)
         ach = byts / avg_s / 1e9
         roof = {"bound": "hbm", "kernel": dom_k, "achieved": round(ach, 1), "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                "unit": "GB/s", "frac": round(ach / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(dom_k, a.precision, a.size),
                 "avg_us": round(avg_s * 1e6, 2), "alg_bytes": int(byts)}
+        if dom_k == "k_nodal_brick":
+            tflops = NODAL_FLOPS_PER_ELEM * E / avg_s / 1e12
+            roof["fp64_tflops"] = round(tflops, 2)
+            roof["fp64_peak_tflops"] = FP64_PEAK_TFLOPS
+            roof["fp64_frac"] = round(tflops / FP64_PEAK_TFLOPS, 4)
 
     extra = {}
     print("[bench] host-loop variant", file=sys.stderr, flush=True)

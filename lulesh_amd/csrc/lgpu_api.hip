@@ -687,9 +687,14 @@ int lgpu_run(lgpu_ctx *c, lgpu_time_state *ts, int32_t maxCycle)
 
    c->s.dtDevice = &c->devTime->deltatime;
    c->s.active = &c->devTime->active;
-   const int batch = 16;
+   // Cycles are enqueued in batches between host checks of the loop flag.
+   // No more cycles are enqueued than the cycle limit allows, so only a run
+   // that ends at stoptime (or on an error) enqueues no-op cycles.
+   int64_t remaining = (int64_t)maxCycle - (int64_t)ts->cycle;
    int st = LGPU_OK;
    for (;;) {
+      const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(16, remaining));
+      remaining -= batch;
       for (int b = 0; b < batch; ++b) {
          st = timed(c, kKTimeIncrement, [&] {
             hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, c->stream, c->devTime,
