@@ -15,6 +15,8 @@ Prints one JSON line.
              (HIP events on the context stream, measured in the timed region)
   cpu_baseline  the reference's own OpenMP build (oracle/_ref/lulesh2.0) on
              this host's cores, on a bounded sample of the same workload
+  fp64fma    the FP64 kernels with fused multiply-adds on the same workload,
+             with the origin-energy error against the reference's number
   mixed      mixed-precision form vs FP64 at s=192, 11 regions, 200 cycles,
              with the origin-energy error against the reference's number
   ensemble   64 Sedov problems (std::mt19937_64(1234) draw), 100 cycles,
@@ -43,7 +45,7 @@ HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
 FP64_PEAK_TFLOPS = 78.6  # MI355X FP64 vector
 # Algorithmic FP64 operations per element of the fused nodal pass (one
 # element's stress integration 376, hourglass control 1379, node update 66;
-# counted from the reference formulas).
+# counted from the reference formulas, lulesh.cc:381-474, 591-706, 1139-1219).
 NODAL_FLOPS_PER_ELEM = 1821
 
 
@@ -143,8 +145,12 @@ def timed_run(la, torch, s, r, b, c, prec, warmup, steps, device_loop, timing=Fa
 
     if timing:
         # Warm-up with every kernel timed: per-kernel breakdown and the
-        # dominant kernel; the timed region then times only that one.
+        # dominant kernel; the timed region then times only that one.  The
+        # first cycle (code-object loading) is left out of the breakdown.
         la.check(lib.lgpu_set_option(ctx, 2, 1))
+        if dom.run(1, device_loop) != 0:
+            raise RuntimeError("warm-up failed")
+        la.check(lib.lgpu_reset_kernel_stats(ctx))
     if dom.run(warmup, device_loop) != 0:
         raise RuntimeError("warm-up failed")
     breakdown = {}
@@ -172,8 +178,8 @@ def mixed_leg(la, torch, s: int, steps: int) -> dict:
     FP64 number where it is recorded."""
     warm = 5
     total = warm + steps
-    fm, msm, dm, _ = timed_run(la, torch, s, 11, 1, 1, la.MIXED, warm, steps, True)
-    fd, msd, dd, _ = timed_run(la, torch, s, 11, 1, 1, la.FP64, warm, steps, True)
+    fm, msm, dm, (kbm, _) = timed_run(la, torch, s, 11, 1, 1, la.MIXED, warm, steps, True, timing=True)
+    fd, msd, dd, (kbd, _) = timed_run(la, torch, s, 11, 1, 1, la.FP64, warm, steps, True, timing=True)
     dm.sync()
     dd.sync()
     e_m, e_d = float(dm.real("e")[0]), float(dd.real("e")[0])
@@ -182,7 +188,9 @@ def mixed_leg(la, torch, s: int, steps: int) -> dict:
            "mixed_zones_per_us": round(fm, 2), "fp64_zones_per_us": round(fd, 2),
            "speedup": round(fm / fd, 3), "cycles": total,
            "e0_mixed": e_m, "e0_fp64": e_d,
-           "e0_rel_err_vs_fp64_gpu": abs(e_m - e_d) / abs(e_d)}
+           "e0_rel_err_vs_fp64_gpu": abs(e_m - e_d) / abs(e_d),
+           "mixed_kernels_ms_per_step": {k: round(v[1] / v[0], 4) for k, v in kbm.items()},
+           "fp64_kernels_ms_per_step": {k: round(v[1] / v[0], 4) for k, v in kbd.items()}}
     if ref is not None:
         out["e0_reference"] = ref
         out["e0_rel_err_vs_reference"] = abs(e_m - ref) / abs(ref)
@@ -190,6 +198,27 @@ def mixed_leg(la, torch, s: int, steps: int) -> dict:
     for f in ("e", "p", "q", "v", "xd"):
         a, b_ = dm.real(f), dd.real(f)
         out[f"linf_{f}"] = float(abs(a - b_).max() / max(abs(b_).max(), 1e-300))
+    return out
+
+
+def fp64fma_leg(la, torch, s: int, r: int, b: int, c: int, fp64_fom: float) -> dict:
+    """The FP64_FMA form (FP64 kernels with fused multiply-adds) on the
+    headline workload: 5 + 195 cycles from cycle 0, so that the final origin
+    energy compares with the reference's 200-cycle number."""
+    warm, steps = 5, 195
+    f, ms, d, (kb, _) = timed_run(la, torch, s, r, b, c, la.FP64_FMA, warm, steps, True, timing=True)
+    d.sync()
+    e0 = float(d.real("e")[0])
+    out = {"workload": f"sedov s={s} r={r} b={b} c={c}, cycles {warm + steps}",
+           "zones_per_us": round(f, 2), "ms_per_step": round(ms, 4),
+           "speedup_vs_fp64": round(f / fp64_fom, 3), "e0": e0,
+           "symmetry": list(d.symmetry()),
+           "kernels_ms_per_step": {k: round(v[1] / v[0], 4) for k, v in kb.items()}}
+    ref = golden_e0(s, r, b, c, warm + steps)
+    if ref is not None:
+        out["e0_reference"] = ref
+        out["e0_rel_err_vs_reference"] = abs(e0 - ref) / abs(ref)
+    d.close()
     return out
 
 
@@ -243,7 +272,7 @@ def main() -> None:
     ap.add_argument("--regions", type=int, default=21)
     ap.add_argument("--balance", type=int, default=2)
     ap.add_argument("--cost", type=int, default=2)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "fp64fma", "mixed"], default="fp64")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-kernel-timing", action="store_true")
@@ -251,6 +280,7 @@ def main() -> None:
                     help="TimeIncrement on the host with a per-cycle dt read-back "
                          "(default: on the device)")
     ap.add_argument("--no-mixed", action="store_true")
+    ap.add_argument("--no-fma", action="store_true", help="skip the FP64_FMA leg")
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -275,7 +305,7 @@ def main() -> None:
         dst.copy_(src)
         torch.cuda.synchronize()
         del src, dst
-    prec = la.MIXED if a.precision == "mixed" else la.FP64
+    prec = {"fp64": la.FP64, "fp64fma": la.FP64_FMA, "mixed": la.MIXED}[a.precision]
     print(f"[bench] s={a.size} r={a.regions} b={a.balance} c={a.cost} {a.precision}",
           file=sys.stderr, flush=True)
     fom, ms_step, dom, (kbreak, kstats) = timed_run(la, torch, a.size, a.regions, a.balance,
@@ -309,6 +339,9 @@ def main() -> None:
                                       a.warmup, min(a.steps, 100), False)
     del dom_h
     extra["host_loop_zones_per_us"] = round(fom_h, 2)
+    if not a.no_fma and a.precision == "fp64":
+        print("[bench] FP64_FMA leg", file=sys.stderr, flush=True)
+        extra["fp64fma"] = fp64fma_leg(la, torch, a.size, a.regions, a.balance, a.cost, fom)
     if not a.no_mixed:
         print("[bench] mixed-precision leg", file=sys.stderr, flush=True)
         extra["mixed"] = mixed_leg(la, torch, a.size, 195)
@@ -333,7 +366,7 @@ def main() -> None:
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if a.precision == "fp64" else "f64+f32",
+        "dtype": "f64+f32" if a.precision == "mixed" else "f64",
         "data": "synthetic (Sedov blast initial conditions, deterministic)",
         "config": {"workload": f"sedov s={a.size} ({E} elements) r={a.regions} b={a.balance} "
                                f"c={a.cost} {a.precision}",

@@ -45,9 +45,13 @@ enum lgpu_status {
 
 enum lgpu_precision {
    LGPU_FP64 = 0,  /* every quantity in FP64, arithmetic as the reference */
-   LGPU_MIXED = 1  /* FP32 force integration, Q gradients, EOS pressure and
+   LGPU_MIXED = 1, /* FP32 force integration, Q gradients, EOS pressure and
                       sound speed; FP64 node integration, volumes, energy
                       update and dt reductions */
+   LGPU_FP64_FMA = 2 /* every quantity in FP64, the reference's association
+                        order, with each multiply-add fused (one rounding
+                        instead of two); results agree with the reference's
+                        to roundoff rather than bit for bit */
 };
 
 /* Real fields (Domain accessors of lulesh.h:266-367).  Node fields first. */

@@ -4,8 +4,8 @@ The device path is hand-written HIP behind a C ABI (include/lulesh_gpu.h);
 the host keeps the reference's Domain class and per-cycle call structure
 (include/lulesh_host.h).  This package only loads those libraries.
 """
-from .native import (FIELDS, FP64, MIXED, DeviceContext, Ensemble, HostDomain,  # noqa: F401
+from .native import (FIELDS, FP64, FP64_FMA, MIXED, DeviceContext, Ensemble, HostDomain,  # noqa: F401
                      TimeState, check, default_constants, ensemble_draw, gpu_lib, host_lib)
 
-__all__ = ["FIELDS", "FP64", "MIXED", "DeviceContext", "Ensemble", "HostDomain", "TimeState",
+__all__ = ["FIELDS", "FP64", "FP64_FMA", "MIXED", "DeviceContext", "Ensemble", "HostDomain", "TimeState",
            "check", "default_constants", "ensemble_draw", "gpu_lib", "host_lib"]

@@ -7,7 +7,8 @@
 
 Every HIP source is compiled for gfx950 only.  The FP64 step kernels and the
 context code are built without FMA contraction so they round as the
-reference's x86-64 build; the mixed-precision kernels allow contraction.
+reference's x86-64 build; the mixed-precision kernels and the FP64_FMA
+instantiation allow contraction.
 """
 from __future__ import annotations
 
@@ -37,6 +38,7 @@ HIP_SOURCES = {
     "lgpu_api.hip": ["-ffp-contract=off"],
     "lgpu_step_fp64.hip": ["-ffp-contract=off"],
     "lgpu_step_mixed.hip": ["-ffp-contract=fast"],
+    "lgpu_step_fp64fma.hip": ["-ffp-contract=fast"],
     "lgpu_ensemble.hip": ["-ffp-contract=off"],
 }
 HOST_SOURCES = ["host/domain.cc", "host/glibc_rand.cc"]

@@ -4,7 +4,7 @@
 //
 // Flags as the reference (-q -i -s -r -b -c -p -h; -f/-v accepted and
 // ignored) plus:
-//   -P fp64|mixed   device precision (default fp64)
+//   -P fp64|fp64fma|mixed   device precision (default fp64)
 //   -D              run TimeIncrement on the device (no per-cycle host sync)
 #include <stdio.h>
 #include <stdlib.h>
@@ -35,7 +35,7 @@ void usage(const char *exe)
    printf(" -b <balance>    : Load balance between regions of a domain (def: 1)\n");
    printf(" -c <cost>       : Extra cost of more expensive regions (def: 1)\n");
    printf(" -p              : Print out progress\n");
-   printf(" -P fp64|mixed   : device precision (def: fp64)\n");
+   printf(" -P fp64|fp64fma|mixed : device precision (def: fp64)\n");
    printf(" -D              : time step control on the device\n");
    printf(" -h              : This message\n");
 }
@@ -75,7 +75,8 @@ Opts parse(int argc, char **argv)
          if (i + 1 >= argc) { printf("Missing argument to -P\n"); exit(-1); }
          if (!strcmp(argv[i + 1], "fp64")) o.precision = LGPU_FP64;
          else if (!strcmp(argv[i + 1], "mixed")) o.precision = LGPU_MIXED;
-         else { printf("-P takes fp64 or mixed\n"); exit(-1); }
+         else if (!strcmp(argv[i + 1], "fp64fma")) o.precision = LGPU_FP64_FMA;
+         else { printf("-P takes fp64, fp64fma or mixed\n"); exit(-1); }
          i += 2;
       } else if (!strcmp(argv[i], "-h")) {
          usage(argv[0]);
@@ -96,7 +97,9 @@ int main(int argc, char **argv)
    const Opts o = parse(argc, argv);
    if (!o.quiet) {
       std::cout << "Running problem size " << o.nx << "^3 per domain until completion\n";
-      std::cout << "Num processors: 1 (one MI355X, " << (o.precision == LGPU_MIXED ? "mixed" : "fp64")
+      std::cout << "Num processors: 1 (one MI355X, " << (o.precision == LGPU_MIXED      ? "mixed"
+                                                    : o.precision == LGPU_FP64_FMA ? "fp64fma"
+                                                                                   : "fp64")
                 << ")\n";
       std::cout << "Total number of elements: " << (long long)o.nx * o.nx * o.nx << " \n\n";
    }

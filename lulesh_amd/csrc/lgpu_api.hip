@@ -155,7 +155,10 @@ int validate(const lgpu_mesh_desc *d, std::string &why)
       why = "missing array";
       return LGPU_EINVAL;
    }
-   if (d->precision != LGPU_FP64 && d->precision != LGPU_MIXED) { why = "precision"; return LGPU_EINVAL; }
+   if (d->precision != LGPU_FP64 && d->precision != LGPU_MIXED && d->precision != LGPU_FP64_FMA) {
+      why = "precision";
+      return LGPU_EINVAL;
+   }
    if (d->numReg < 1 || d->cost < 0) { why = "numReg/cost"; return LGPU_EINVAL; }
    if (!in_range(d->nodelist, 8 * E, 0, (int32_t)N)) { why = "nodelist index out of range"; return LGPU_EINVAL; }
    const int32_t *nb[6] = {d->lxim, d->lxip, d->letam, d->letap, d->lzetam, d->lzetap};
@@ -248,6 +251,21 @@ int32_t region_rep(int32_t r, int32_t numReg, int32_t cost)
    if (r < numReg / 2) return 1;
    if (r < numReg - (numReg + 15) / 20) return 1 + cost;
    return 10 * (1 + cost);
+}
+
+// Order in which the regions are laid out for k_eos: largest EOS repeat
+// count first (stable within a count), so that the longest-running
+// workgroups are dispatched first and the short ones fill in behind them.
+// An element's EOS does not depend on where it sits in the list, and the dt
+// minimum does not depend on the order of its candidates.
+std::vector<int32_t> heavy_first(int32_t numReg, int32_t cost)
+{
+   std::vector<int32_t> order((size_t)numReg);
+   for (int32_t r = 0; r < numReg; ++r) order[(size_t)r] = r;
+   std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+      return region_rep(a, numReg, cost) > region_rep(b, numReg, cost);
+   });
+   return order;
 }
 
 double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &isGrad)
@@ -532,10 +550,12 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    std::vector<int32_t> perm;
    perm.reserve(E);
    std::vector<int32_t> regStart((size_t)d->numReg + 1, 0), regRep((size_t)d->numReg, 1);
-   for (int r = 0; r < d->numReg; ++r) {
-      regStart[(size_t)r] = (int32_t)perm.size();
+   const std::vector<int32_t> order = heavy_first(d->numReg, d->cost);
+   for (int i = 0; i < d->numReg; ++i) {
+      const int32_t r = order[(size_t)i];
+      regStart[(size_t)i] = (int32_t)perm.size();
       perm.insert(perm.end(), d->regElemlist[r], d->regElemlist[r] + d->regElemSize[r]);
-      regRep[(size_t)r] = region_rep(r, d->numReg, d->cost);
+      regRep[(size_t)i] = region_rep(r, d->numReg, d->cost);
    }
    regStart[(size_t)d->numReg] = (int32_t)perm.size();
    // Empty regions sit at the same offset as the next one; the binary search
@@ -631,15 +651,21 @@ int lgpu_download(lgpu_ctx *c, int field, double *host)
 int lgpu_lagrange_nodal(lgpu_ctx *c, double dt)
 {
    if (!c) return LGPU_EINVAL;
-   return (c->precision == LGPU_MIXED) ? launch_nodal<PolicyMixed>(c, dt)
-                                       : launch_nodal<PolicyFp64>(c, dt);
+   switch (c->precision) {
+   case LGPU_MIXED: return launch_nodal<PolicyMixed>(c, dt);
+   case LGPU_FP64_FMA: return launch_nodal<PolicyFp64Fma>(c, dt);
+   default: return launch_nodal<PolicyFp64>(c, dt);
+   }
 }
 
 int lgpu_lagrange_elements(lgpu_ctx *c, double dt)
 {
    if (!c) return LGPU_EINVAL;
-   return (c->precision == LGPU_MIXED) ? launch_elements<PolicyMixed>(c, dt)
-                                       : launch_elements<PolicyFp64>(c, dt);
+   switch (c->precision) {
+   case LGPU_MIXED: return launch_elements<PolicyMixed>(c, dt);
+   case LGPU_FP64_FMA: return launch_elements<PolicyFp64Fma>(c, dt);
+   default: return launch_elements<PolicyFp64>(c, dt);
+   }
 }
 
 int lgpu_time_constraints(lgpu_ctx *c, double *dtcourant, double *dthydro)

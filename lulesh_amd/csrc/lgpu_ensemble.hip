@@ -139,8 +139,11 @@ void launch_cycle(lgpu_ensemble *e, int maxCycle)
 
 void enqueue_cycle(lgpu_ensemble *e, int maxCycle)
 {
-   if (e->precision == LGPU_MIXED) launch_cycle<PolicyMixed>(e, maxCycle);
-   else launch_cycle<PolicyFp64>(e, maxCycle);
+   switch (e->precision) {
+   case LGPU_MIXED: launch_cycle<PolicyMixed>(e, maxCycle); break;
+   case LGPU_FP64_FMA: launch_cycle<PolicyFp64Fma>(e, maxCycle); break;
+   default: launch_cycle<PolicyFp64>(e, maxCycle); break;
+   }
 }
 
 // Per-kernel timing of one cycle (used by the autotuner and the timing
@@ -175,8 +178,11 @@ int timed_cycle(lgpu_ensemble *e, int maxCycle, float ms[kENumKernels])
 
 int timed_cycle_any(lgpu_ensemble *e, int maxCycle, float ms[kENumKernels])
 {
-   return e->precision == LGPU_MIXED ? timed_cycle<PolicyMixed>(e, maxCycle, ms)
-                                     : timed_cycle<PolicyFp64>(e, maxCycle, ms);
+   switch (e->precision) {
+   case LGPU_MIXED: return timed_cycle<PolicyMixed>(e, maxCycle, ms);
+   case LGPU_FP64_FMA: return timed_cycle<PolicyFp64Fma>(e, maxCycle, ms);
+   default: return timed_cycle<PolicyFp64>(e, maxCycle, ms);
+   }
 }
 
 void drop_graph(lgpu_ensemble *e)
@@ -245,6 +251,7 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
    if (!members || nMembers < 1 || !out) return LGPU_EINVAL;
    *out = nullptr;
    const int32_t prec = members[0].precision;
+   if (prec != LGPU_FP64 && prec != LGPU_MIXED && prec != LGPU_FP64_FMA) return LGPU_EINVAL;
    for (int32_t m = 0; m < nMembers; ++m) {
       const lgpu_mesh_desc &d = members[m];
       if (d.precision != prec || memcmp(&d.k, &members[0].k, sizeof(lgpu_constants)) != 0) {
@@ -415,9 +422,17 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
       for (int a = 0; a < 3; ++a)
          for (int32_t i = 0; i < ns[a]; ++i) bc[(size_t)(n0 + sy[a][i])] |= (uint8_t)(1u << a);
       int64_t pos = (int64_t)blockStart[m] * kEosBlock;
-      for (int32_t r = 0; r < d.numReg; ++r) {
-         regStart[(size_t)(regBase + r)] = (int32_t)pos;
-         regRep[(size_t)(regBase + r)] = region_rep(r, d.numReg, d.cost);
+      // Largest EOS repeat count first within the member (as for a single
+      // domain): long workgroups start first, short ones fill in behind.
+      std::vector<int32_t> order((size_t)d.numReg);
+      for (int32_t r = 0; r < d.numReg; ++r) order[(size_t)r] = r;
+      std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
+         return region_rep(a, d.numReg, d.cost) > region_rep(b, d.numReg, d.cost);
+      });
+      for (int32_t i = 0; i < d.numReg; ++i) {
+         const int32_t r = order[(size_t)i];
+         regStart[(size_t)(regBase + i)] = (int32_t)pos;
+         regRep[(size_t)(regBase + i)] = region_rep(r, d.numReg, d.cost);
          for (int32_t i = 0; i < d.regElemSize[r]; ++i) perm[(size_t)pos++] = d.regElemlist[r][i] + e0;
       }
       regBase += d.numReg;

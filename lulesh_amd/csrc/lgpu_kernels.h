@@ -50,6 +50,11 @@ struct PolicyFp64 {
    static constexpr bool kLocalCoords = false;
 };
 
+// The FP64 policy compiled with multiply-add contraction (its own
+// translation unit); the types are those of PolicyFp64.
+struct PolicyFp64Fma : PolicyFp64 {
+};
+
 struct PolicyMixed {
    using force_t = float;
    using corner_t = float;

@@ -30,5 +30,6 @@ struct Launch {
 
 extern template struct Launch<PolicyFp64>;
 extern template struct Launch<PolicyMixed>;
+extern template struct Launch<PolicyFp64Fma>;
 
 } // namespace lgpu

@@ -22,7 +22,7 @@ FIELDS = [
     "elemMass", "delx_xi", "delx_eta", "delx_zeta", "delv_xi", "delv_eta", "delv_zeta",
 ]
 NODE_FIELDS = set(FIELDS[:13])
-FP64, MIXED = 0, 1
+FP64, MIXED, FP64_FMA = 0, 1, 2
 OPT_KEEP_DIAGNOSTICS, OPT_KERNEL_TIMING, OPT_BLOCK_SIZE, OPT_BRICK_CHUNK, OPT_XCD_REMAP = 1, 2, 3, 4, 5
 FLAG_GENERIC = 1
 STATUS = {0: "ok", -1: "VolumeError", -2: "QStopError", -100: "EINVAL", -101: "EDEVICE",

@@ -7,8 +7,8 @@
   oracle must reproduce iteration counts, final origin energy and the
   symmetry figures exactly.
 * The single-threaded reference build sums forces in element order; its s=30
-  result (2.02507478507199383e+05, recorded from that build in BASELINE.md's
-  measurements) pins the oracle's scatter order.
+  result (2.02507478507199383e+05, from a one-thread run of that build)
+  pins the oracle's scatter order.
 """
 import numpy as np
 import pytest
