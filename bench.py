@@ -68,6 +68,7 @@ def kernel_bytes(name: str, s: int, precision: str, brick: bool = True) -> float
         "k_nodal_brick": 48 * r + 48 + 8 * r + 1 * r + 48 * r,
         # nodelist 32, x..zd 48r, volo v 16 -> vnew delv arealg vdov 32, grads 6
         "k_kin": nl + 48 * r + 16 + 32 + 6 * gb,
+        "k_kin_brick": 48 * r + 16 + 32 + 6 * gb,
         # perm 4, elemBC 4, 6 nbr 24, grads 6, 10 fields 80 -> 5 fields 40
         "k_eos": 4 + 4 + 24 + 6 * gb + 80 + 40,
     }

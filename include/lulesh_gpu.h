@@ -164,10 +164,15 @@ enum lgpu_option {
                                      brick kernel (1..mesh edge)             */
    LGPU_OPT_XCD_REMAP = 5,        /* 1 (default): contiguous block ranges per
                                      XCD; 0: hardware order                 */
-   LGPU_OPT_KERNEL_TIMING_MASK = 6 /* bit k: time kernel k while timing is on
+   LGPU_OPT_KERNEL_TIMING_MASK = 6, /* bit k: time kernel k while timing is on
                                      (default: all)                          */
+   LGPU_OPT_KIN_CHUNK = 7         /* element planes per workgroup of the
+                                     brick kinematics kernel (1..mesh edge)  */
 };
 int lgpu_set_option(lgpu_ctx *ctx, int option, int64_t value);
+/* Current value of an option (the brick chunks report the defaults chosen at
+ * lgpu_create: from the CU count and the kernels' occupancy). */
+int lgpu_get_option(lgpu_ctx *ctx, int option, int64_t *value);
 
 /* Kernel timing (LGPU_OPT_KERNEL_TIMING).  Kernels are numbered
  * 0..lgpu_num_kernels()-1. */

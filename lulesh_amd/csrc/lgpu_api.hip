@@ -23,11 +23,12 @@ using namespace lgpu;
 namespace {
 
 enum KernelId {
-   kKForce = 0, kKNode, kKKin, kKEos, kKDtReduce, kKTimeIncrement, kKNodalBrick, kNumKernels
+   kKForce = 0, kKNode, kKKin, kKEos, kKDtReduce, kKTimeIncrement, kKNodalBrick, kKKinBrick,
+   kNumKernels
 };
 const char *const kKernelNames[kNumKernels] = {"k_force", "k_node", "k_kin", "k_eos",
                                                "k_dt_reduce", "k_time_increment",
-                                               "k_nodal_brick"};
+                                               "k_nodal_brick", "k_kin_brick"};
 
 constexpr size_t kAlign = 256;
 
@@ -245,6 +246,35 @@ void set_brick_chunk(DevState &s, int chunk)
    g.chunks = (g.en + g.chunkZ - 1) / g.chunkZ;
 }
 
+// Default node planes per workgroup of the brick kernel.  A workgroup walks
+// chunk + 1 element planes for chunk node planes, and the launch runs in
+// rounds of `slots` resident workgroups; pick the chunk that minimises
+// rounds x (chunk + 1), i.e. redundant planes against a partly filled last
+// round (s=192 on 256 CUs: 13).
+int pick_brick_chunk(const Brick &g, int slots)
+{
+   if (slots < 1) return 8;
+   int best = 8;
+   double bestCost = 1e300;
+   for (int ch = 4; ch <= 32 && ch <= g.en; ++ch) {
+      const int64_t blocks = (int64_t)g.tilesX * g.tilesY * ((g.en + ch - 1) / ch);
+      const int64_t rounds = (blocks + slots - 1) / slots;
+      const double cost = (double)rounds * (ch + 1);
+      if (cost < bestCost) {
+         bestCost = cost;
+         best = ch;
+      }
+   }
+   return best;
+}
+
+void set_kin_chunk(DevState &s, int chunk)
+{
+   Brick &g = s.brick;
+   g.kinChunk = std::max(1, std::min(chunk, g.s));
+   g.kinChunks = (g.s + g.kinChunk - 1) / g.kinChunk;
+}
+
 // EOS repeat count of region r, lulesh.cc:2390-2400.
 int32_t region_rep(int32_t r, int32_t numReg, int32_t cost)
 {
@@ -350,7 +380,9 @@ int launch_nodal(lgpu_ctx *c, double dt)
 template <class P>
 int launch_elements(lgpu_ctx *c, double dt)
 {
-   int r = timed(c, kKKin, [&] { Launch<P>::kin(c->s, c->k, dt, c->block, c->stream); });
+   int r = c->s.isBrick
+              ? timed(c, kKKinBrick, [&] { Launch<P>::kin_brick(c->s, c->k, dt, c->stream); })
+              : timed(c, kKKin, [&] { Launch<P>::kin(c->s, c->k, dt, c->block, c->stream); });
    if (r) return r;
    r = timed(c, kKEos, [&] { Launch<P>::eos(c->s, c->k, c->stream); });
    if (r) return r;
@@ -538,7 +570,18 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
       g.tilesX = g.tilesY = (g.en + 14) / 15;
       g.divS = make_fastdiv((uint32_t)g.s);
       g.divS2 = make_fastdiv((uint32_t)(g.s * g.s));
-      set_brick_chunk(s, 8);
+      g.kinTiles = (g.s + 14) / 15;
+      int cus = 0;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+         cus = 0;
+      int perCu = 0;
+      switch (d->precision) {
+      case LGPU_MIXED: perCu = Launch<PolicyMixed>::nodal_brick_blocks_per_cu(); break;
+      case LGPU_FP64_FMA: perCu = Launch<PolicyFp64Fma>::nodal_brick_blocks_per_cu(); break;
+      default: perCu = Launch<PolicyFp64>::nodal_brick_blocks_per_cu(); break;
+      }
+      set_brick_chunk(s, pick_brick_chunk(g, cus * perCu));
+      set_kin_chunk(s, 8);
    }
 
    // Host-built tables.
@@ -787,11 +830,36 @@ int lgpu_set_option(lgpu_ctx *c, int option, int64_t value)
    case LGPU_OPT_XCD_REMAP:
       c->s.xcdRemap = value ? 1 : 0;
       return LGPU_OK;
+   case LGPU_OPT_KIN_CHUNK:
+      if (!c->s.isBrick || value < 1) return LGPU_EINVAL;
+      set_kin_chunk(c->s, (int)std::min<int64_t>(value, 1 << 20));
+      return LGPU_OK;
    case LGPU_OPT_KERNEL_TIMING_MASK:
       c->timingMask = (uint32_t)value;
       return LGPU_OK;
    default:
       return LGPU_EINVAL;
+   }
+}
+
+int lgpu_get_option(lgpu_ctx *c, int option, int64_t *value)
+{
+   if (!c || !value) return LGPU_EINVAL;
+   switch (option) {
+   case LGPU_OPT_KEEP_DIAGNOSTICS: *value = c->keepDiag ? 1 : 0; return LGPU_OK;
+   case LGPU_OPT_KERNEL_TIMING: *value = c->timing ? 1 : 0; return LGPU_OK;
+   case LGPU_OPT_BLOCK_SIZE: *value = c->block; return LGPU_OK;
+   case LGPU_OPT_BRICK_CHUNK:
+      if (!c->s.isBrick) return LGPU_EINVAL;
+      *value = c->s.brick.chunkZ;
+      return LGPU_OK;
+   case LGPU_OPT_XCD_REMAP: *value = c->s.xcdRemap; return LGPU_OK;
+   case LGPU_OPT_KERNEL_TIMING_MASK: *value = c->timingMask; return LGPU_OK;
+   case LGPU_OPT_KIN_CHUNK:
+      if (!c->s.isBrick) return LGPU_EINVAL;
+      *value = c->s.brick.kinChunk;
+      return LGPU_OK;
+   default: return LGPU_EINVAL;
    }
 }
 

@@ -47,7 +47,10 @@ struct PolicyFp64 {
    using corner_t = double; // stored per-corner force
    using grad_t = double;   // Q gradients + limiter arithmetic and storage
    using eos_t = double;    // EOS pressure / sound-speed arithmetic
+   using nodesum_t = double; // per-node sums of the corner forces
    static constexpr bool kLocalCoords = false;
+   // brick kernel: load the node update's operands before the element work
+   static constexpr bool kPrefetchNode = true;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
@@ -60,7 +63,9 @@ struct PolicyMixed {
    using corner_t = float;
    using grad_t = float;
    using eos_t = float;
+   using nodesum_t = float;
    static constexpr bool kLocalCoords = true;
+   static constexpr bool kPrefetchNode = true;
 };
 
 // Division by a run-time constant d (< 2^32) as a 64-bit multiply-high:
@@ -78,6 +83,8 @@ struct Brick {
    int32_t tilesX, tilesY; // 15x15-node column tiles
    int32_t chunkZ, chunks; // node planes per z-chunk, number of chunks
    FastDiv divS, divS2;    // element index -> (a, b, c)
+   // k_kin_brick: 15x15-element column tiles, element planes per z-chunk
+   int32_t kinTiles, kinChunk, kinChunks;
 };
 
 // Everything a cycle touches on the device.

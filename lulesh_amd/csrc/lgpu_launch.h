@@ -1,5 +1,6 @@
 // lgpu_launch.h -- host-side launchers of the step kernels, one explicit
-// instantiation per precision policy (lgpu_step_fp64.hip, lgpu_step_mixed.hip).
+// instantiation per precision policy (lgpu_step_fp64.hip, lgpu_step_mixed.hip,
+// lgpu_step_fp64fma.hip).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -23,7 +24,10 @@ struct Launch {
    static void node(const DevState &s, const Consts &k, int64_t planeStride, double dt, int block,
                     hipStream_t st);
    static void nodal_brick(const DevState &s, const Consts &k, double dt, hipStream_t st);
+   // resident k_nodal_brick workgroups per CU (0 if the query fails)
+   static int nodal_brick_blocks_per_cu();
    static void kin(const DevState &s, const Consts &k, double dt, int block, hipStream_t st);
+   static void kin_brick(const DevState &s, const Consts &k, double dt, hipStream_t st);
    static void eos(const DevState &s, const Consts &k, hipStream_t st);
    static void dt_reduce(const DevState &s, hipStream_t st);
 };

@@ -24,6 +24,7 @@ FIELDS = [
 NODE_FIELDS = set(FIELDS[:13])
 FP64, MIXED, FP64_FMA = 0, 1, 2
 OPT_KEEP_DIAGNOSTICS, OPT_KERNEL_TIMING, OPT_BLOCK_SIZE, OPT_BRICK_CHUNK, OPT_XCD_REMAP = 1, 2, 3, 4, 5
+OPT_KERNEL_TIMING_MASK, OPT_KIN_CHUNK = 6, 7
 FLAG_GENERIC = 1
 STATUS = {0: "ok", -1: "VolumeError", -2: "QStopError", -100: "EINVAL", -101: "EDEVICE",
           -102: "ENOMEM"}
@@ -101,6 +102,8 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lgpu_run.restype = ctypes.c_int
         lib.lgpu_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int64]
         lib.lgpu_set_option.restype = ctypes.c_int
+        lib.lgpu_get_option.argtypes = [vp, ctypes.c_int, ctypes.POINTER(ctypes.c_int64)]
+        lib.lgpu_get_option.restype = ctypes.c_int
         lib.lgpu_num_kernels.argtypes = []
         lib.lgpu_num_kernels.restype = ctypes.c_int
         lib.lgpu_kernel_name.argtypes = [ctypes.c_int]
@@ -280,6 +283,11 @@ class DeviceContext:
 
     def set_option(self, option: int, value: int) -> None:
         check(gpu_lib().lgpu_set_option(self.ptr, option, value), self.ptr)
+
+    def get_option(self, option: int) -> int:
+        v = ctypes.c_int64()
+        check(gpu_lib().lgpu_get_option(self.ptr, option, ctypes.byref(v)), self.ptr)
+        return v.value
 
     def lagrange_nodal(self, dt: float) -> int:
         return check(gpu_lib().lgpu_lagrange_nodal(self.ptr, dt), self.ptr)

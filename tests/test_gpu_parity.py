@@ -89,13 +89,16 @@ def test_lagrange_elements_and_dt(s, r, b, c, warm, generic):
     assert dth == o.get("dthydro")
 
 
-@pytest.mark.parametrize("s,chunk", [(7, 8), (16, 3), (33, 5)])
-def test_brick_and_generic_paths_bit_identical(s, chunk):
-    """Both nodal paths sum corners in the reference's order: whole cycles
-    agree bit for bit, whatever the brick kernel's z-chunking."""
+@pytest.mark.parametrize("s,chunk,kin_chunk", [(7, 8, 8), (16, 3, 1), (33, 5, 7), (31, 16, 31)])
+def test_brick_and_generic_paths_bit_identical(s, chunk, kin_chunk):
+    """Both paths sum corners in the reference's order and evaluate the
+    kinematics from the same node values: whole cycles agree bit for bit,
+    whatever the brick kernels' tiling and z-chunking (meshes that are not a
+    multiple of the 15-wide tiles included)."""
     o = OracleDomain(s, 11, 1, 1)
     ctxs = [context_from_oracle(o, generic=g) for g in (False, True)]
-    ctxs[0].set_option(4, chunk)
+    ctxs[0].set_option(la.native.OPT_BRICK_CHUNK, chunk)
+    ctxs[0].set_option(la.native.OPT_KIN_CHUNK, kin_chunk)
     for ctx in ctxs:
         upload_state(ctx, o)
     dt = o.get("deltatime")
@@ -215,3 +218,21 @@ def test_volume_error_code():
     st, _, _ = ctx.time_constraints()
     assert st == -1
     assert o.call("lagrange_leapfrog") == -1
+
+
+def test_brick_options_report_and_override():
+    """The brick kernels' chunk defaults are chosen at create time (from the
+    CU count and occupancy) and can be read back and overridden."""
+    o = OracleDomain(40, 11, 1, 1)
+    ctx = context_from_oracle(o)
+    ch = ctx.get_option(la.native.OPT_BRICK_CHUNK)
+    kin = ctx.get_option(la.native.OPT_KIN_CHUNK)
+    print(f"s=40 defaults: brick chunk {ch}, kin chunk {kin}")
+    assert 4 <= ch <= 32 and 1 <= kin <= 40
+    ctx.set_option(la.native.OPT_BRICK_CHUNK, 5)
+    ctx.set_option(la.native.OPT_KIN_CHUNK, 3)
+    assert ctx.get_option(la.native.OPT_BRICK_CHUNK) == 5
+    assert ctx.get_option(la.native.OPT_KIN_CHUNK) == 3
+    generic = context_from_oracle(o, generic=True)
+    with pytest.raises(RuntimeError):
+        generic.get_option(la.native.OPT_BRICK_CHUNK)
