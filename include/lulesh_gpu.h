@@ -182,10 +182,14 @@ int lgpu_kernel_stats(lgpu_ctx *ctx, int kernel, int64_t *launches, double *tota
 int lgpu_reset_kernel_stats(lgpu_ctx *ctx);
 
 /* Context facts for reports: path (0 generic CSR gather, 1 structured
- * brick), device arena bytes, precision. */
+ * brick), device arena bytes, precision; brick_faces = 1 when the face
+ * neighbours and boundary masks are those of the brick with the reference's
+ * boundary conditions, so that the EOS pass computes them instead of reading
+ * them. */
 typedef struct lgpu_info {
    int32_t path, precision, numElem, numNode, numReg, block;
    int64_t arena_bytes;
+   int32_t brick_faces, reserved;
 } lgpu_info;
 int lgpu_get_info(lgpu_ctx *ctx, lgpu_info *info);
 

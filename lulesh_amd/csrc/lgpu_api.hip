@@ -239,6 +239,34 @@ bool is_reference_brick(const lgpu_mesh_desc *d)
    return true;
 }
 
+// True when the face neighbours and boundary masks are those of the brick
+// with symmetry planes on the low faces and free surfaces on the high faces
+// (the reference's single-domain set-up, lulesh-init.cc:340-398, 497-573), so
+// that k_eos can compute them (brick_faces in lgpu_step.inc).  A neighbour
+// index has to match only where the mask lets k_eos read it (bits 0 or COMM).
+bool brick_faces_computable(const lgpu_mesh_desc *d)
+{
+   const int64_t s = d->sizeX, s2 = s * s;
+   const int32_t *nb[6] = {d->lxim, d->lxip, d->letam, d->letap, d->lzetam, d->lzetap};
+   const int32_t field[6] = {0x00007, 0x00038, 0x001c0, 0x00e00, 0x07000, 0x38000};
+   const int32_t comm[6] = {0x00004, 0x00020, 0x00100, 0x00800, 0x04000, 0x20000};
+   int64_t e = 0;
+   for (int64_t c = 0; c < s; ++c)
+      for (int64_t b = 0; b < s; ++b)
+         for (int64_t a = 0; a < s; ++a, ++e) {
+            const int32_t want = (a == 0 ? 0x00001 : 0) | (a == s - 1 ? 0x00010 : 0) |
+                                 (b == 0 ? 0x00040 : 0) | (b == s - 1 ? 0x00400 : 0) |
+                                 (c == 0 ? 0x01000 : 0) | (c == s - 1 ? 0x10000 : 0);
+            if (d->elemBC[e] != want) return false;
+            const int64_t wantN[6] = {e - 1, e + 1, e - s, e + s, e - s2, e + s2};
+            for (int f = 0; f < 6; ++f) {
+               const int32_t bits = want & field[f];
+               if ((bits == 0 || bits == comm[f]) && nb[f][e] != wantN[f]) return false;
+            }
+         }
+   return true;
+}
+
 void set_brick_chunk(DevState &s, int chunk)
 {
    Brick &g = s.brick;
@@ -563,6 +591,7 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    s.numPerm = d->numElem;
    s.xcdRemap = 1;
    s.isBrick = brick ? 1 : 0;
+   s.brickFaces = (brick && brick_faces_computable(d)) ? 1 : 0;
    if (brick) {
       Brick &g = s.brick;
       g.s = d->sizeX;
@@ -902,6 +931,8 @@ int lgpu_get_info(lgpu_ctx *c, lgpu_info *info)
    info->numReg = c->numReg;
    info->block = c->block;
    info->arena_bytes = (int64_t)c->arenaBytes;
+   info->brick_faces = c->s.brickFaces;
+   info->reserved = 0;
    return LGPU_OK;
 }
 

@@ -48,6 +48,7 @@ struct PolicyFp64 {
    using grad_t = double;   // Q gradients + limiter arithmetic and storage
    using eos_t = double;    // EOS pressure / sound-speed arithmetic
    using nodesum_t = double; // per-node sums of the corner forces
+   using kin_t = double;     // characteristic length and strain rate
    static constexpr bool kLocalCoords = false;
    // brick kernel: load the node update's operands before the element work
    static constexpr bool kPrefetchNode = true;
@@ -64,6 +65,7 @@ struct PolicyMixed {
    using grad_t = float;
    using eos_t = float;
    using nodesum_t = float;
+   using kin_t = double;
    static constexpr bool kLocalCoords = true;
    static constexpr bool kPrefetchNode = true;
 };
@@ -119,6 +121,7 @@ struct DevState {
    const int32_t *active;   // device-side loop flag (null: always active)
    Brick brick;             // valid when isBrick
    int32_t isBrick;
+   int32_t brickFaces; // k_eos computes face neighbours and BC masks from the brick
    // Ensemble of independent problems (nMembers > 0): members are
    // concatenated; each has its own dt, loop flag and error flags.
    int32_t nMembers;

@@ -68,7 +68,8 @@ class Info(ctypes.Structure):
     _fields_ = [("path", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("numElem", ctypes.c_int32), ("numNode", ctypes.c_int32),
                 ("numReg", ctypes.c_int32), ("block", ctypes.c_int32),
-                ("arena_bytes", ctypes.c_int64)]
+                ("arena_bytes", ctypes.c_int64), ("brick_faces", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
 
 
 _GPU: ctypes.CDLL | None = None

@@ -51,6 +51,9 @@ def test_lagrange_nodal(s, r, b, c, warm, zero_ss, generic):
         o.real("ss")[:] = 0.0
     ctx = context_from_oracle(o, generic=generic)
     assert ctx.info().path == (0 if generic else 1)
+    # the reference's boundary set-up is recognised, so the brick EOS pass
+    # computes face neighbours and masks instead of reading them
+    assert ctx.info().brick_faces == (0 if generic else 1)
     ctx.set_option(1, 1)  # keep fx.., xdd..
     upload_state(ctx, o)
     dt = o.get("deltatime")
