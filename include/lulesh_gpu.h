@@ -185,11 +185,12 @@ int lgpu_reset_kernel_stats(lgpu_ctx *ctx);
  * brick), device arena bytes, precision; brick_faces = 1 when the face
  * neighbours and boundary masks are those of the brick with the reference's
  * boundary conditions, so that the EOS pass computes them instead of reading
- * them. */
+ * them; brick_blocks_per_cu = resident workgroups of the fused nodal kernel
+ * per CU, as used for the default chunk. */
 typedef struct lgpu_info {
    int32_t path, precision, numElem, numNode, numReg, block;
    int64_t arena_bytes;
-   int32_t brick_faces, reserved;
+   int32_t brick_faces, brick_blocks_per_cu;
 } lgpu_info;
 int lgpu_get_info(lgpu_ctx *ctx, lgpu_info *info);
 

@@ -55,6 +55,7 @@ struct lgpu_ctx {
    int32_t sizeX = 0;
    int block = 256;
    bool keepDiag = false;
+   int32_t brickPerCu = 0;
    bool timing = false;
    uint32_t timingMask = 0xffffffffu;
    Consts k{};
@@ -326,10 +327,13 @@ std::vector<int32_t> heavy_first(int32_t numReg, int32_t cost)
    return order;
 }
 
-double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &isGrad)
+// Device array of a field.  `narrow`: stored as FP32 in the mixed form (the
+// Q gradients, p, q and ss).
+double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &narrow)
 {
    isElem = f >= LGPU_E;
-   isGrad = f >= LGPU_DELX_XI;
+   narrow = c->precision == LGPU_MIXED &&
+            (f >= LGPU_DELX_XI || f == LGPU_P || f == LGPU_Q || f == LGPU_SS);
    DevState &s = c->s;
    switch (f) {
    case LGPU_X: return s.x;
@@ -346,8 +350,8 @@ double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &isGrad)
    case LGPU_FZ: return s.fz;
    case LGPU_NODALMASS: return s.nodalMass;
    case LGPU_E: return s.e;
-   case LGPU_P: return s.p;
-   case LGPU_Q: return s.q;
+   case LGPU_P: return static_cast<double *>(s.p);
+   case LGPU_Q: return static_cast<double *>(s.q);
    case LGPU_QL: return s.ql;
    case LGPU_QQ: return s.qq;
    case LGPU_V: return s.v;
@@ -356,7 +360,7 @@ double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &isGrad)
    case LGPU_DELV: return s.delv;
    case LGPU_VDOV: return s.vdov;
    case LGPU_AREALG: return s.arealg;
-   case LGPU_SS: return s.ss;
+   case LGPU_SS: return static_cast<double *>(s.ss);
    case LGPU_ELEMMASS: return s.elemMass;
    case LGPU_DELX_XI: return static_cast<double *>(s.delx_xi);
    case LGPU_DELX_ETA: return static_cast<double *>(s.delx_eta);
@@ -558,7 +562,9 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    add((void **)&s.nodelist, sizeof(int32_t) * 8 * E);
    const int32_t **nbr[] = {&s.lxim, &s.lxip, &s.letam, &s.letap, &s.lzetam, &s.lzetap, &s.elemBC};
    for (const int32_t **p : nbr) add((void **)p, sizeof(int32_t) * E);
-   double **elemReal[] = {&s.e, &s.p, &s.q, &s.v, &s.volo, &s.ss, &s.elemMass,
+   void **pqsArrays[] = {&s.p, &s.q, &s.ss};
+   for (void **p : pqsArrays) add(p, gradBytes * E);
+   double **elemReal[] = {&s.e, &s.v, &s.volo, &s.elemMass,
                           &s.vnew, &s.delv, &s.arealg, &s.vdov};
    for (double **p : elemReal) add((void **)p, sizeof(double) * E);
    void **grads[] = {&s.delx_xi, &s.delx_eta, &s.delx_zeta, &s.delv_xi, &s.delv_eta, &s.delv_zeta};
@@ -609,6 +615,7 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
       case LGPU_FP64_FMA: perCu = Launch<PolicyFp64Fma>::nodal_brick_blocks_per_cu(); break;
       default: perCu = Launch<PolicyFp64>::nodal_brick_blocks_per_cu(); break;
       }
+      c->brickPerCu = perCu;
       set_brick_chunk(s, pick_brick_chunk(g, cus * perCu));
       set_kin_chunk(s, 8);
    }
@@ -680,16 +687,16 @@ void lgpu_destroy(lgpu_ctx *c)
 int lgpu_upload(lgpu_ctx *c, int field, const double *host)
 {
    if (!c || !host || field < 0 || field >= LGPU_NUM_FIELDS) return LGPU_EINVAL;
-   bool isElem, isGrad;
+   bool isElem, narrow;
    if ((field >= LGPU_XDD && field <= LGPU_FZ) || field == LGPU_QL || field == LGPU_QQ) {
       int r = enable_diagnostics(c);
       if (r) return r;
    }
-   double *dst = real_field_ptr(c, field, isElem, isGrad);
+   double *dst = real_field_ptr(c, field, isElem, narrow);
    if (!dst) return LGPU_EINVAL;
    const size_t n = isElem ? (size_t)c->numElem : (size_t)c->numNode;
    HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
-   if (isGrad && c->precision == LGPU_MIXED) {
+   if (narrow) {
       std::vector<float> tmp(n);
       for (size_t i = 0; i < n; ++i) tmp[i] = (float)host[i];
       HIP_OR_RETURN(c, hipMemcpy(dst, tmp.data(), sizeof(float) * n, hipMemcpyHostToDevice));
@@ -702,15 +709,15 @@ int lgpu_upload(lgpu_ctx *c, int field, const double *host)
 int lgpu_download(lgpu_ctx *c, int field, double *host)
 {
    if (!c || !host || field < 0 || field >= LGPU_NUM_FIELDS) return LGPU_EINVAL;
-   bool isElem, isGrad;
-   double *src = real_field_ptr(c, field, isElem, isGrad);
+   bool isElem, narrow;
+   double *src = real_field_ptr(c, field, isElem, narrow);
    const size_t n = isElem ? (size_t)c->numElem : (size_t)c->numNode;
    HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
    if (!src) {
       std::fill(host, host + n, 0.0);
       return LGPU_OK;
    }
-   if (isGrad && c->precision == LGPU_MIXED) {
+   if (narrow) {
       std::vector<float> tmp(n);
       HIP_OR_RETURN(c, hipMemcpy(tmp.data(), src, sizeof(float) * n, hipMemcpyDeviceToHost));
       for (size_t i = 0; i < n; ++i) host[i] = (double)tmp[i];
@@ -932,7 +939,7 @@ int lgpu_get_info(lgpu_ctx *c, lgpu_info *info)
    info->block = c->block;
    info->arena_bytes = (int64_t)c->arenaBytes;
    info->brick_faces = c->s.brickFaces;
-   info->reserved = 0;
+   info->brick_blocks_per_cu = c->brickPerCu;
    return LGPU_OK;
 }
 

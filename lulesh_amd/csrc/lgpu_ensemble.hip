@@ -208,10 +208,13 @@ int build_graph(lgpu_ensemble *e, int maxCycle)
    return LGPU_OK;
 }
 
-double *field_base(lgpu_ensemble *e, int f, bool &isElem, bool &isGrad)
+// Device array of a field.  `narrow`: stored as FP32 in the mixed form (the
+// Q gradients, p, q and ss).
+double *field_base(lgpu_ensemble *e, int f, bool &isElem, bool &narrow)
 {
    isElem = f >= LGPU_E;
-   isGrad = f >= LGPU_DELX_XI;
+   narrow = e->precision == LGPU_MIXED &&
+            (f >= LGPU_DELX_XI || f == LGPU_P || f == LGPU_Q || f == LGPU_SS);
    DevState &s = e->s;
    switch (f) {
    case LGPU_X: return s.x;
@@ -222,15 +225,15 @@ double *field_base(lgpu_ensemble *e, int f, bool &isElem, bool &isGrad)
    case LGPU_ZD: return s.zd;
    case LGPU_NODALMASS: return s.nodalMass;
    case LGPU_E: return s.e;
-   case LGPU_P: return s.p;
-   case LGPU_Q: return s.q;
+   case LGPU_P: return static_cast<double *>(s.p);
+   case LGPU_Q: return static_cast<double *>(s.q);
    case LGPU_V: return s.v;
    case LGPU_VOLO: return s.volo;
    case LGPU_VNEW: return s.vnew;
    case LGPU_DELV: return s.delv;
    case LGPU_VDOV: return s.vdov;
    case LGPU_AREALG: return s.arealg;
-   case LGPU_SS: return s.ss;
+   case LGPU_SS: return static_cast<double *>(s.ss);
    case LGPU_ELEMMASS: return s.elemMass;
    case LGPU_DELX_XI: return static_cast<double *>(s.delx_xi);
    case LGPU_DELX_ETA: return static_cast<double *>(s.delx_eta);
@@ -338,8 +341,10 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
    auto add = [&](void **dst, size_t bytes) { slots.push_back({dst, align_up(bytes)}); };
    double **stateNode[] = {&s.x, &s.y, &s.z, &s.xd, &s.yd, &s.zd};
    for (double **p : stateNode) add((void **)p, 8 * N);
-   double **stateElem[] = {&s.e, &s.p, &s.q, &s.v, &s.ss};
+   double **stateElem[] = {&s.e, &s.v};
    for (double **p : stateElem) add((void **)p, 8 * E);
+   void **statePqs[] = {&s.p, &s.q, &s.ss};
+   for (void **p : statePqs) add(p, gb * E);
    EnsTime &t = e->t;
    double **tf[] = {&t.time, &t.deltatime, &t.dtcourant, &t.dthydro, &t.stoptime, &t.dtfixed,
                     &t.dtmax, &t.multlb, &t.multub};
@@ -483,13 +488,13 @@ void lgpu_ensemble_destroy(lgpu_ensemble *e)
 int lgpu_ensemble_upload(lgpu_ensemble *e, int32_t member, int field, const double *host)
 {
    if (!e || !host || member < 0 || member >= e->nMembers) return LGPU_EINVAL;
-   bool isElem, isGrad;
-   double *base = field_base(e, field, isElem, isGrad);
+   bool isElem, narrow;
+   double *base = field_base(e, field, isElem, narrow);
    if (!base) return LGPU_EINVAL;
    const size_t n = isElem ? (size_t)e->elemCount[member] : (size_t)e->nodeCount[member];
    const size_t off = isElem ? (size_t)e->elemStart[member] : (size_t)e->nodeStart[member];
    ENS_HIP(e, hipStreamSynchronize(e->stream));
-   if (isGrad && e->precision == LGPU_MIXED) {
+   if (narrow) {
       std::vector<float> tmp(n);
       for (size_t i = 0; i < n; ++i) tmp[i] = (float)host[i];
       ENS_HIP(e, hipMemcpy(reinterpret_cast<float *>(base) + off, tmp.data(), 4 * n, hipMemcpyHostToDevice));
@@ -502,13 +507,13 @@ int lgpu_ensemble_upload(lgpu_ensemble *e, int32_t member, int field, const doub
 int lgpu_ensemble_download(lgpu_ensemble *e, int32_t member, int field, double *host)
 {
    if (!e || !host || member < 0 || member >= e->nMembers) return LGPU_EINVAL;
-   bool isElem, isGrad;
-   double *base = field_base(e, field, isElem, isGrad);
+   bool isElem, narrow;
+   double *base = field_base(e, field, isElem, narrow);
    if (!base) return LGPU_EINVAL;
    const size_t n = isElem ? (size_t)e->elemCount[member] : (size_t)e->nodeCount[member];
    const size_t off = isElem ? (size_t)e->elemStart[member] : (size_t)e->nodeStart[member];
    ENS_HIP(e, hipStreamSynchronize(e->stream));
-   if (isGrad && e->precision == LGPU_MIXED) {
+   if (narrow) {
       std::vector<float> tmp(n);
       ENS_HIP(e, hipMemcpy(tmp.data(), reinterpret_cast<float *>(base) + off, 4 * n, hipMemcpyDeviceToHost));
       for (size_t i = 0; i < n; ++i) host[i] = tmp[i];

@@ -49,6 +49,7 @@ struct PolicyFp64 {
    using eos_t = double;    // EOS pressure / sound-speed arithmetic
    using nodesum_t = double; // per-node sums of the corner forces
    using kin_t = double;     // characteristic length and strain rate
+   using pqs_t = double;     // stored pressure, artificial viscosity, sound speed
    static constexpr bool kLocalCoords = false;
    // brick kernel: load the node update's operands before the element work
    static constexpr bool kPrefetchNode = true;
@@ -66,6 +67,7 @@ struct PolicyMixed {
    using eos_t = float;
    using nodesum_t = float;
    using kin_t = double;
+   using pqs_t = float;
    static constexpr bool kLocalCoords = true;
    static constexpr bool kPrefetchNode = true;
 };
@@ -102,7 +104,8 @@ struct DevState {
    // element fields
    const int32_t *nodelist;
    const int32_t *lxim, *lxip, *letam, *letap, *lzetam, *lzetap, *elemBC;
-   double *e, *p, *q, *v, *volo, *ss, *elemMass;
+   double *e, *v, *volo, *elemMass;
+   void *p, *q, *ss; // P::pqs_t: FP64, or FP32 in the mixed form
    double *vnew, *delv, *arealg, *vdov;
    double *ql, *qq; // diagnostics (may be null)
    void *delx_xi, *delx_eta, *delx_zeta, *delv_xi, *delv_eta, *delv_zeta; // grad_t

@@ -69,7 +69,7 @@ class Info(ctypes.Structure):
                 ("numElem", ctypes.c_int32), ("numNode", ctypes.c_int32),
                 ("numReg", ctypes.c_int32), ("block", ctypes.c_int32),
                 ("arena_bytes", ctypes.c_int64), ("brick_faces", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("brick_blocks_per_cu", ctypes.c_int32)]
 
 
 _GPU: ctypes.CDLL | None = None

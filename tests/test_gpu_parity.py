@@ -230,7 +230,8 @@ def test_brick_options_report_and_override():
     ctx = context_from_oracle(o)
     ch = ctx.get_option(la.native.OPT_BRICK_CHUNK)
     kin = ctx.get_option(la.native.OPT_KIN_CHUNK)
-    print(f"s=40 defaults: brick chunk {ch}, kin chunk {kin}")
+    print(f"s=40 defaults: brick chunk {ch}, kin chunk {kin}, "
+          f"nodal workgroups per CU {ctx.info().brick_blocks_per_cu}")
     assert 4 <= ch <= 32 and 1 <= kin <= 40
     ctx.set_option(la.native.OPT_BRICK_CHUNK, 5)
     ctx.set_option(la.native.OPT_KIN_CHUNK, 3)
