@@ -166,8 +166,11 @@ enum lgpu_option {
                                      XCD; 0: hardware order                 */
    LGPU_OPT_KERNEL_TIMING_MASK = 6, /* bit k: time kernel k while timing is on
                                      (default: all)                          */
-   LGPU_OPT_KIN_CHUNK = 7         /* element planes per workgroup of the
+   LGPU_OPT_KIN_CHUNK = 7,        /* element planes per workgroup of the
                                      brick kinematics kernel (1..mesh edge)  */
+   LGPU_OPT_BRICK_ROWS = 8        /* thread rows of a fused brick workgroup:
+                                     16 (16x16 element columns, default) or 8
+                                     (16x8)                                  */
 };
 int lgpu_set_option(lgpu_ctx *ctx, int option, int64_t value);
 /* Current value of an option (the brick chunks report the defaults chosen at

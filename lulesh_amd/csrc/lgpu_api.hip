@@ -606,6 +606,7 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
       g.divS = make_fastdiv((uint32_t)g.s);
       g.divS2 = make_fastdiv((uint32_t)(g.s * g.s));
       g.kinTiles = (g.s + 14) / 15;
+      g.rows = 16;
       int cus = 0;
       if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
          cus = 0;
@@ -870,6 +871,10 @@ int lgpu_set_option(lgpu_ctx *c, int option, int64_t value)
       if (!c->s.isBrick || value < 1) return LGPU_EINVAL;
       set_kin_chunk(c->s, (int)std::min<int64_t>(value, 1 << 20));
       return LGPU_OK;
+   case LGPU_OPT_BRICK_ROWS:
+      if (!c->s.isBrick || (value != 8 && value != 16)) return LGPU_EINVAL;
+      c->s.brick.rows = (int32_t)value;
+      return LGPU_OK;
    case LGPU_OPT_KERNEL_TIMING_MASK:
       c->timingMask = (uint32_t)value;
       return LGPU_OK;
@@ -894,6 +899,10 @@ int lgpu_get_option(lgpu_ctx *c, int option, int64_t *value)
    case LGPU_OPT_KIN_CHUNK:
       if (!c->s.isBrick) return LGPU_EINVAL;
       *value = c->s.brick.kinChunk;
+      return LGPU_OK;
+   case LGPU_OPT_BRICK_ROWS:
+      if (!c->s.isBrick) return LGPU_EINVAL;
+      *value = c->s.brick.rows;
       return LGPU_OK;
    default: return LGPU_EINVAL;
    }

@@ -86,6 +86,7 @@ struct Brick {
    int32_t s, en;          // elements and nodes per edge
    int32_t tilesX, tilesY; // 15x15-node column tiles
    int32_t chunkZ, chunks; // node planes per z-chunk, number of chunks
+   int32_t rows;           // thread rows per k_nodal_brick workgroup: 16 (256 threads) or 8
    FastDiv divS, divS2;    // element index -> (a, b, c)
    // k_kin_brick: 15x15-element column tiles, element planes per z-chunk
    int32_t kinTiles, kinChunk, kinChunks;

@@ -92,8 +92,9 @@ def test_lagrange_elements_and_dt(s, r, b, c, warm, generic):
     assert dth == o.get("dthydro")
 
 
-@pytest.mark.parametrize("s,chunk,kin_chunk", [(7, 8, 8), (16, 3, 1), (33, 5, 7), (31, 16, 31)])
-def test_brick_and_generic_paths_bit_identical(s, chunk, kin_chunk):
+@pytest.mark.parametrize("s,chunk,kin_chunk,rows", [(7, 8, 8, 16), (16, 3, 1, 16), (33, 5, 7, 16),
+                                                    (31, 16, 31, 16), (17, 6, 4, 8), (30, 13, 8, 8)])
+def test_brick_and_generic_paths_bit_identical(s, chunk, kin_chunk, rows):
     """Both paths sum corners in the reference's order and evaluate the
     kinematics from the same node values: whole cycles agree bit for bit,
     whatever the brick kernels' tiling and z-chunking (meshes that are not a
@@ -102,6 +103,7 @@ def test_brick_and_generic_paths_bit_identical(s, chunk, kin_chunk):
     ctxs = [context_from_oracle(o, generic=g) for g in (False, True)]
     ctxs[0].set_option(la.native.OPT_BRICK_CHUNK, chunk)
     ctxs[0].set_option(la.native.OPT_KIN_CHUNK, kin_chunk)
+    ctxs[0].set_option(la.native.OPT_BRICK_ROWS, rows)
     for ctx in ctxs:
         upload_state(ctx, o)
     dt = o.get("deltatime")
