@@ -56,23 +56,24 @@ def kernel_bytes(name: str, s: int, precision: str, brick: bool = True) -> float
     node map, face neighbours and BC masks are computed, not read."""
     E = s ** 3
     r = ((s + 1) / s) ** 3
-    n4 = 4 if precision == "mixed" else 8   # corner forces, Q gradients, p q ss
+    # FP32 in the mixed form: corner forces, Q gradients, p q ss elemMass arealg vdov
+    n4 = 4 if precision == "mixed" else 8
     nl = 0 if brick else 32                 # element-to-node map
     faces = 0 if brick else 28              # elemBC + 6 face neighbours
     per_elem = {
-        # nodelist 32, x..zd 48r, p q ss, v volo elemMass 24 -> 48 corner values
-        "k_force": 32 + 48 * r + 3 * n4 + 24 + 48 * n4,
+        # nodelist 32, x..zd 48r, p q ss elemMass, v volo 16 -> 48 corner values
+        "k_force": 32 + 48 * r + 4 * n4 + 16 + 48 * n4,
         # start 4r, corner list 32, 48 corner values, mass 8r, bc 1r, x..zd r/w 96r
         "k_node": 4 * r + 32 + 48 * n4 + 8 * r + 1 * r + 96 * r,
-        # fused brick nodal pass: x..zd read 48r, p q ss, v volo elemMass 24,
+        # fused brick nodal pass: x..zd read 48r, p q ss elemMass, v volo 16,
         # nodalMass 8r, node BC mask 1r, x..zd written 48r
-        "k_nodal_brick": 48 * r + 3 * n4 + 24 + 8 * r + 1 * r + 48 * r,
-        # nodelist, x..zd 48r, volo v 16 -> vnew delv arealg vdov 32, 6 gradients
-        "k_kin": nl + 48 * r + 16 + 32 + 6 * n4,
-        "k_kin_brick": 48 * r + 16 + 32 + 6 * n4,
-        # perm 4, faces, 6 gradients, p q, vdov elemMass volo vnew v e delv
-        # arealg 64 -> p q ss 3, e v 16
-        "k_eos": 4 + faces + 6 * n4 + 2 * n4 + 64 + 3 * n4 + 16,
+        "k_nodal_brick": 48 * r + 4 * n4 + 16 + 8 * r + 1 * r + 48 * r,
+        # nodelist, x..zd 48r, volo v 16 -> vnew delv 16, arealg vdov, 6 gradients
+        "k_kin": nl + 48 * r + 16 + 16 + 8 * n4,
+        "k_kin_brick": 48 * r + 16 + 16 + 8 * n4,
+        # perm 4, faces, 6 gradients, p q vdov elemMass arealg, volo vnew v e
+        # delv 40 -> p q ss, e v 16
+        "k_eos": 4 + faces + 6 * n4 + 5 * n4 + 40 + 3 * n4 + 16,
     }
     return per_elem.get(name, 0.0) * E
 

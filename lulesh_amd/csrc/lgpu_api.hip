@@ -328,12 +328,13 @@ std::vector<int32_t> heavy_first(int32_t numReg, int32_t cost)
 }
 
 // Device array of a field.  `narrow`: stored as FP32 in the mixed form (the
-// Q gradients, p, q and ss).
+// Q gradients, p, q, ss, elemMass, arealg and vdov).
 double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &narrow)
 {
    isElem = f >= LGPU_E;
    narrow = c->precision == LGPU_MIXED &&
-            (f >= LGPU_DELX_XI || f == LGPU_P || f == LGPU_Q || f == LGPU_SS);
+            (f >= LGPU_DELX_XI || f == LGPU_P || f == LGPU_Q || f == LGPU_SS ||
+             f == LGPU_ELEMMASS || f == LGPU_AREALG || f == LGPU_VDOV);
    DevState &s = c->s;
    switch (f) {
    case LGPU_X: return s.x;
@@ -358,10 +359,10 @@ double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &narrow)
    case LGPU_VOLO: return s.volo;
    case LGPU_VNEW: return s.vnew;
    case LGPU_DELV: return s.delv;
-   case LGPU_VDOV: return s.vdov;
-   case LGPU_AREALG: return s.arealg;
+   case LGPU_VDOV: return static_cast<double *>(s.vdov);
+   case LGPU_AREALG: return static_cast<double *>(s.arealg);
    case LGPU_SS: return static_cast<double *>(s.ss);
-   case LGPU_ELEMMASS: return s.elemMass;
+   case LGPU_ELEMMASS: return static_cast<double *>(s.elemMass);
    case LGPU_DELX_XI: return static_cast<double *>(s.delx_xi);
    case LGPU_DELX_ETA: return static_cast<double *>(s.delx_eta);
    case LGPU_DELX_ZETA: return static_cast<double *>(s.delx_zeta);
@@ -562,10 +563,9 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    add((void **)&s.nodelist, sizeof(int32_t) * 8 * E);
    const int32_t **nbr[] = {&s.lxim, &s.lxip, &s.letam, &s.letap, &s.lzetam, &s.lzetap, &s.elemBC};
    for (const int32_t **p : nbr) add((void **)p, sizeof(int32_t) * E);
-   void **pqsArrays[] = {&s.p, &s.q, &s.ss};
+   void **pqsArrays[] = {&s.p, &s.q, &s.ss, &s.elemMass, &s.arealg, &s.vdov};
    for (void **p : pqsArrays) add(p, gradBytes * E);
-   double **elemReal[] = {&s.e, &s.v, &s.volo, &s.elemMass,
-                          &s.vnew, &s.delv, &s.arealg, &s.vdov};
+   double **elemReal[] = {&s.e, &s.v, &s.volo, &s.vnew, &s.delv};
    for (double **p : elemReal) add((void **)p, sizeof(double) * E);
    void **grads[] = {&s.delx_xi, &s.delx_eta, &s.delx_zeta, &s.delv_xi, &s.delv_eta, &s.delv_zeta};
    for (void **p : grads) add(p, gradBytes * E);

@@ -16,6 +16,16 @@ namespace lgpu {
 
 #define LG_DEV __device__ __forceinline__
 
+// Reciprocal, square root and a / sqrt(x): correctly rounded in FP64 (the
+// reference's operations); the hardware approximations (about 1 ulp) in the
+// FP32 arithmetic of the mixed form.
+LG_DEV double recip(double x) { return 1.0 / x; }
+LG_DEV float recip(float x) { return __builtin_amdgcn_rcpf(x); }
+LG_DEV double root(double x) { return sqrt(x); }
+LG_DEV float root(float x) { return __builtin_amdgcn_sqrtf(x); }
+LG_DEV double div_root(double a, double x) { return a / sqrt(x); }
+LG_DEV float div_root(float a, float x) { return a * __builtin_amdgcn_rsqf(x); }
+
 // u . (v x w); the triple product of lulesh.cc:1337-1338 as it is applied
 // at lulesh.cc:1341-1349.
 template <typename T>
@@ -203,7 +213,7 @@ LG_DEV T char_length(const T (&x)[8], const T (&y)[8], const T (&z)[8], T volume
    m = keep_max(face_area2<T, 1, 2, 6, 5>(x, y, z), m);
    m = keep_max(face_area2<T, 2, 3, 7, 6>(x, y, z), m);
    m = keep_max(face_area2<T, 3, 0, 4, 7>(x, y, z), m);
-   return T(4.0) * volume / sqrt(m);
+   return div_root(T(4.0) * volume, m);
 }
 
 // Difference of two four-node sums ((a0+a1)+a2)+a3 - (((b0+b1)+b2)+b3), the

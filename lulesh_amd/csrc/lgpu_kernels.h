@@ -49,7 +49,7 @@ struct PolicyFp64 {
    using eos_t = double;    // EOS pressure / sound-speed arithmetic
    using nodesum_t = double; // per-node sums of the corner forces
    using kin_t = double;     // characteristic length and strain rate
-   using pqs_t = double;     // stored pressure, artificial viscosity, sound speed
+   using pqs_t = double;     // stored p, q, ss, elemMass, arealg, vdov
    static constexpr bool kLocalCoords = false;
    // brick kernel: load the node update's operands before the element work
    static constexpr bool kPrefetchNode = true;
@@ -105,9 +105,11 @@ struct DevState {
    // element fields
    const int32_t *nodelist;
    const int32_t *lxim, *lxip, *letam, *letap, *lzetam, *lzetap, *elemBC;
-   double *e, *v, *volo, *elemMass;
-   void *p, *q, *ss; // P::pqs_t: FP64, or FP32 in the mixed form
-   double *vnew, *delv, *arealg, *vdov;
+   double *e, *v, *volo;
+   // P::pqs_t (FP64, or FP32 in the mixed form): p, q, ss, elemMass, arealg, vdov
+   void *p, *q, *ss, *elemMass;
+   double *vnew, *delv;
+   void *arealg, *vdov;
    double *ql, *qq; // diagnostics (may be null)
    void *delx_xi, *delx_eta, *delx_zeta, *delv_xi, *delv_eta, *delv_zeta; // grad_t
    // corner planes: [(component*2 + kind)*8 + corner][elem], kind 0 stress, 1 hourglass
