@@ -210,7 +210,10 @@ const char *lgpu_last_error(lgpu_ctx *ctx);
  * (lulesh.cc:166-222) runs per member on the device, so members of different
  * sizes advance with their own dt.  One cycle of every member is captured as
  * one hipGraph and replayed.  Results of each member equal a single-problem
- * run of that member (the generic CSR-gather kernels are used).
+ * run of that member.  When every member is the reference's brick (and
+ * members[0].flags lacks LGPU_FLAG_GENERIC) the fused brick kernels run over
+ * all members' workgroups in one launch each; otherwise the CSR-gather
+ * kernels are used.
  * ---------------------------------------------------------------------- */
 typedef struct lgpu_ensemble lgpu_ensemble;
 
@@ -231,7 +234,12 @@ enum lgpu_ensemble_option {
    LGPU_ENS_OPT_BLOCK_FORCE = 3,  /* threads per block: 64, 128 or 256          */
    LGPU_ENS_OPT_BLOCK_NODE = 4,
    LGPU_ENS_OPT_BLOCK_KIN = 5,
-   LGPU_ENS_OPT_XCD_REMAP = 6
+   LGPU_ENS_OPT_XCD_REMAP = 6,
+   LGPU_ENS_OPT_BRICK_CHUNK = 7,  /* brick path: node planes per workgroup of
+                                     the fused nodal kernel, every member      */
+   LGPU_ENS_OPT_KIN_CHUNK = 8,    /* brick path: element planes per
+                                     workgroup of the kinematics kernel       */
+   LGPU_ENS_OPT_PATH = 9          /* read only: 1 brick path, 0 CSR gathers   */
 };
 int lgpu_ensemble_set_option(lgpu_ensemble *ens, int option, int64_t value);
 int lgpu_ensemble_get_option(lgpu_ensemble *ens, int option, int64_t *value);

@@ -81,6 +81,11 @@ void lh_ensemble_draw(uint64_t seed, int32_t n, int32_t *s, double *emult, int32
 /* Builds n host Domains (nr[i] regions each) and their device ensemble. */
 lh_ensemble *lh_ensemble_create(int32_t n, const int32_t *s, const double *emult, const int32_t *nr,
                                 const int32_t *balance, const int32_t *cost, int32_t precision);
+/* The same with lgpu_mesh_desc.flags for the device ensemble
+ * (LGPU_FLAG_GENERIC: CSR-gather kernels instead of the brick path). */
+lh_ensemble *lh_ensemble_create_flags(int32_t n, const int32_t *s, const double *emult,
+                                      const int32_t *nr, const int32_t *balance, const int32_t *cost,
+                                      int32_t precision, int32_t flags);
 void lh_ensemble_destroy(lh_ensemble *e);
 
 /* Runs every member's main loop to cycle its (or t_stop); status optional

@@ -673,6 +673,13 @@ void lh_ensemble_draw(uint64_t seed, int32_t n, int32_t *s, double *emult, int32
 lh_ensemble *lh_ensemble_create(int32_t n, const int32_t *s, const double *emult, const int32_t *nr,
                                 const int32_t *balance, const int32_t *cost, int32_t precision)
 {
+   return lh_ensemble_create_flags(n, s, emult, nr, balance, cost, precision, 0);
+}
+
+lh_ensemble *lh_ensemble_create_flags(int32_t n, const int32_t *s, const double *emult,
+                                      const int32_t *nr, const int32_t *balance, const int32_t *cost,
+                                      int32_t precision, int32_t flags)
+{
    if (n < 1 || !s || !emult || !nr || !balance || !cost) return nullptr;
    for (int32_t i = 0; i < n; ++i)
       if (s[i] < 1 || nr[i] < 1 || cost[i] < 0) return nullptr;
@@ -684,7 +691,10 @@ lh_ensemble *lh_ensemble_create(int32_t n, const int32_t *s, const double *emult
    for (auto &m : e->members) e->views.push_back(lh_domain{m.get(), false});
    std::vector<std::vector<const Index_t *>> lists((size_t)n);
    std::vector<lgpu_mesh_desc> descs;
-   for (int32_t i = 0; i < n; ++i) descs.push_back(describe(*e->members[(size_t)i], lists[(size_t)i], precision));
+   for (int32_t i = 0; i < n; ++i) {
+      descs.push_back(describe(*e->members[(size_t)i], lists[(size_t)i], precision));
+      descs.back().flags = flags;
+   }
    if (lgpu_ensemble_create(descs.data(), n, &e->dev) != LGPU_OK) {
       delete e;
       return nullptr;

@@ -14,6 +14,7 @@
 #include <string>
 #include <vector>
 
+#include "lgpu_brick.h"
 #include "lgpu_kernels.h"
 #include "lgpu_launch.h"
 #include "lulesh_gpu.h"
@@ -193,115 +194,6 @@ int validate(const lgpu_mesh_desc *d, std::string &why)
    }
    if (total != E) { why = "regions do not cover every element"; return LGPU_EINVAL; }
    return LGPU_OK;
-}
-
-FastDiv make_fastdiv(uint32_t d)
-{
-   FastDiv f;
-   f.d = d;
-   f.m = UINT64_MAX / d + 1;
-   return f;
-}
-
-// True when the description is the reference's structured brick: lattice
-// element-to-node map (lulesh-init.cc:245-266) and node-to-corner lists in
-// ascending corner order (lulesh-init.cc:271-319).
-bool is_reference_brick(const lgpu_mesh_desc *d)
-{
-   const int64_t s = d->sizeX;
-   if (s < 1 || d->sizeY != s || d->sizeZ != s) return false;
-   const int64_t en = s + 1;
-   if (d->numElem != s * s * s || d->numNode != en * en * en) return false;
-   int64_t e = 0;
-   for (int64_t c = 0; c < s; ++c)
-      for (int64_t b = 0; b < s; ++b)
-         for (int64_t a = 0; a < s; ++a, ++e) {
-            const int64_t n0 = (c * en + b) * en + a;
-            const int64_t want[8] = {n0, n0 + 1, n0 + en + 1, n0 + en, n0 + en * en,
-                                     n0 + en * en + 1, n0 + en * en + en + 1, n0 + en * en + en};
-            for (int j = 0; j < 8; ++j)
-               if (d->nodelist[8 * e + j] != want[j]) return false;
-         }
-   // Corner lists: every node's corners in ascending order.
-   const int64_t N = d->numNode;
-   std::vector<int32_t> cnt((size_t)N, 0);
-   for (int64_t c = 0; c < 8 * (int64_t)d->numElem; ++c) cnt[(size_t)d->nodelist[c]]++;
-   int64_t start = 0;
-   for (int64_t n = 0; n < N; ++n) {
-      if (d->nodeElemStart[n] != start) return false;
-      start += cnt[(size_t)n];
-   }
-   if (d->nodeElemStart[N] != start) return false;
-   std::fill(cnt.begin(), cnt.end(), 0);
-   for (int64_t c = 0; c < 8 * (int64_t)d->numElem; ++c) {
-      const int32_t n = d->nodelist[c];
-      if (d->nodeElemCornerList[d->nodeElemStart[n] + cnt[(size_t)n]++] != (int32_t)c) return false;
-   }
-   return true;
-}
-
-// True when the face neighbours and boundary masks are those of the brick
-// with symmetry planes on the low faces and free surfaces on the high faces
-// (the reference's single-domain set-up, lulesh-init.cc:340-398, 497-573), so
-// that k_eos can compute them (brick_faces in lgpu_step.inc).  A neighbour
-// index has to match only where the mask lets k_eos read it (bits 0 or COMM).
-bool brick_faces_computable(const lgpu_mesh_desc *d)
-{
-   const int64_t s = d->sizeX, s2 = s * s;
-   const int32_t *nb[6] = {d->lxim, d->lxip, d->letam, d->letap, d->lzetam, d->lzetap};
-   const int32_t field[6] = {0x00007, 0x00038, 0x001c0, 0x00e00, 0x07000, 0x38000};
-   const int32_t comm[6] = {0x00004, 0x00020, 0x00100, 0x00800, 0x04000, 0x20000};
-   int64_t e = 0;
-   for (int64_t c = 0; c < s; ++c)
-      for (int64_t b = 0; b < s; ++b)
-         for (int64_t a = 0; a < s; ++a, ++e) {
-            const int32_t want = (a == 0 ? 0x00001 : 0) | (a == s - 1 ? 0x00010 : 0) |
-                                 (b == 0 ? 0x00040 : 0) | (b == s - 1 ? 0x00400 : 0) |
-                                 (c == 0 ? 0x01000 : 0) | (c == s - 1 ? 0x10000 : 0);
-            if (d->elemBC[e] != want) return false;
-            const int64_t wantN[6] = {e - 1, e + 1, e - s, e + s, e - s2, e + s2};
-            for (int f = 0; f < 6; ++f) {
-               const int32_t bits = want & field[f];
-               if ((bits == 0 || bits == comm[f]) && nb[f][e] != wantN[f]) return false;
-            }
-         }
-   return true;
-}
-
-void set_brick_chunk(DevState &s, int chunk)
-{
-   Brick &g = s.brick;
-   g.chunkZ = std::max(1, std::min(chunk, g.en));
-   g.chunks = (g.en + g.chunkZ - 1) / g.chunkZ;
-}
-
-// Default node planes per workgroup of the brick kernel.  A workgroup walks
-// chunk + 1 element planes for chunk node planes, and the launch runs in
-// rounds of `slots` resident workgroups; pick the chunk that minimises
-// rounds x (chunk + 1), i.e. redundant planes against a partly filled last
-// round (s=192 on 256 CUs: 13).
-int pick_brick_chunk(const Brick &g, int slots)
-{
-   if (slots < 1) return 8;
-   int best = 8;
-   double bestCost = 1e300;
-   for (int ch = 4; ch <= 32 && ch <= g.en; ++ch) {
-      const int64_t blocks = (int64_t)g.tilesX * g.tilesY * ((g.en + ch - 1) / ch);
-      const int64_t rounds = (blocks + slots - 1) / slots;
-      const double cost = (double)rounds * (ch + 1);
-      if (cost < bestCost) {
-         bestCost = cost;
-         best = ch;
-      }
-   }
-   return best;
-}
-
-void set_kin_chunk(DevState &s, int chunk)
-{
-   Brick &g = s.brick;
-   g.kinChunk = std::max(1, std::min(chunk, g.s));
-   g.kinChunks = (g.s + g.kinChunk - 1) / g.kinChunk;
 }
 
 // EOS repeat count of region r, lulesh.cc:2390-2400.
@@ -600,13 +492,7 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    s.brickFaces = (brick && brick_faces_computable(d)) ? 1 : 0;
    if (brick) {
       Brick &g = s.brick;
-      g.s = d->sizeX;
-      g.en = d->sizeX + 1;
-      g.tilesX = g.tilesY = (g.en + 14) / 15;
-      g.divS = make_fastdiv((uint32_t)g.s);
-      g.divS2 = make_fastdiv((uint32_t)(g.s * g.s));
-      g.kinTiles = (g.s + 14) / 15;
-      g.rows = 16;
+      init_brick(g, d->sizeX);
       int cus = 0;
       if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
          cus = 0;
@@ -617,8 +503,7 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
       default: perCu = Launch<PolicyFp64>::nodal_brick_blocks_per_cu(); break;
       }
       c->brickPerCu = perCu;
-      set_brick_chunk(s, pick_brick_chunk(g, cus * perCu));
-      set_kin_chunk(s, 8);
+      set_brick_chunk(g, pick_brick_chunk(g, cus * perCu));
    }
 
    // Host-built tables.
@@ -862,14 +747,14 @@ int lgpu_set_option(lgpu_ctx *c, int option, int64_t value)
       return LGPU_OK;
    case LGPU_OPT_BRICK_CHUNK:
       if (!c->s.isBrick || value < 1) return LGPU_EINVAL;
-      set_brick_chunk(c->s, (int)std::min<int64_t>(value, 1 << 20));
+      set_brick_chunk(c->s.brick, (int)std::min<int64_t>(value, 1 << 20));
       return LGPU_OK;
    case LGPU_OPT_XCD_REMAP:
       c->s.xcdRemap = value ? 1 : 0;
       return LGPU_OK;
    case LGPU_OPT_KIN_CHUNK:
       if (!c->s.isBrick || value < 1) return LGPU_EINVAL;
-      set_kin_chunk(c->s, (int)std::min<int64_t>(value, 1 << 20));
+      set_kin_chunk(c->s.brick, (int)std::min<int64_t>(value, 1 << 20));
       return LGPU_OK;
    case LGPU_OPT_BRICK_ROWS:
       if (!c->s.isBrick || (value != 8 && value != 16)) return LGPU_EINVAL;

@@ -16,6 +16,7 @@
 #include <string>
 #include <vector>
 
+#include "lgpu_brick.h"
 #include "lgpu_kernels.h"
 #include "lgpu_launch.h"
 #include "lulesh_gpu.h"
@@ -24,9 +25,12 @@ using namespace lgpu;
 
 namespace {
 
-enum EnsKernel { kEForce = 0, kENode, kEKin, kEEos, kEDt, kETime, kENumKernels };
+enum EnsKernel {
+   kEForce = 0, kENode, kEKin, kEEos, kEDt, kETime, kENodalBrick, kEKinBrick, kENumKernels
+};
 const char *const kEnsKernelNames[kENumKernels] = {"k_force", "k_node", "k_kin", "k_eos",
-                                                   "k_dt_reduce", "k_time_increment"};
+                                                   "k_dt_reduce", "k_time_increment",
+                                                   "k_nodal_brick", "k_kin_brick"};
 
 constexpr size_t kAlign = 256;
 inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
@@ -103,6 +107,19 @@ struct lgpu_ensemble {
    hipGraph_t graph = nullptr;
    hipGraphExec_t exec = nullptr;
    int graphMaxCycle = -1;
+   // Brick path: every member is a reference brick; the fused nodal and
+   // kinematics kernels run over all members' workgroups, node state
+   // ping-pongs between two buffers (parity = which one is current), and a
+   // cycle graph is captured for each parity.
+   bool brick = false;
+   int parity = 0;
+   hipGraph_t graph2 = nullptr;
+   hipGraphExec_t exec2 = nullptr;
+   std::vector<int32_t> memberSize;
+   int nodalChunk = 8, kinChunk = 8;
+   unsigned nodalBlocks = 0, kinBlocks = 0;
+   char *tables = nullptr;
+   size_t tablesBytes = 0;
    int64_t kLaunches[kENumKernels] = {};
    double kMs[kENumKernels] = {};
    std::string lastError;
@@ -125,14 +142,32 @@ int fail_hip(lgpu_ensemble *e, hipError_t err, const char *what)
       if (err_ != hipSuccess) return fail_hip((e), err_, #call);    \
    } while (0)
 
+void swap_nodes(lgpu_ensemble *e)
+{
+   DevState &s = e->s;
+   std::swap(s.x, s.xo);
+   std::swap(s.y, s.yo);
+   std::swap(s.z, s.zo);
+   std::swap(s.xd, s.xdo);
+   std::swap(s.yd, s.ydo);
+   std::swap(s.zd, s.zdo);
+   e->parity ^= 1;
+}
+
 template <class P>
 void launch_cycle(lgpu_ensemble *e, int maxCycle)
 {
    hipLaunchKernelGGL(k_time_increment_ens, dim3((e->nMembers + 63) / 64), dim3(64), 0, e->stream,
                       e->t, e->s.memberDtOut, e->s.memberErr, e->nMembers, maxCycle);
-   Launch<P>::force(e->s, e->k, e->planeStride, e->block[0], e->stream);
-   Launch<P>::node(e->s, e->k, e->planeStride, 0.0, e->block[1], e->stream);
-   Launch<P>::kin(e->s, e->k, 0.0, e->block[2], e->stream);
+   if (e->brick) {
+      Launch<P>::nodal_brick_members(e->s, e->k, e->nodalBlocks, e->stream);
+      swap_nodes(e);
+      Launch<P>::kin_brick_members(e->s, e->k, e->kinBlocks, e->stream);
+   } else {
+      Launch<P>::force(e->s, e->k, e->planeStride, e->block[0], e->stream);
+      Launch<P>::node(e->s, e->k, e->planeStride, 0.0, e->block[1], e->stream);
+      Launch<P>::kin(e->s, e->k, 0.0, e->block[2], e->stream);
+   }
    Launch<P>::eos(e->s, e->k, e->stream);
    Launch<P>::dt_reduce(e->s, e->stream);
 }
@@ -157,11 +192,19 @@ int timed_cycle(lgpu_ensemble *e, int maxCycle, float ms[kENumKernels])
    hipLaunchKernelGGL(k_time_increment_ens, dim3((e->nMembers + 63) / 64), dim3(64), 0, e->stream,
                       e->t, e->s.memberDtOut, e->s.memberErr, e->nMembers, maxCycle);
    ENS_HIP(e, hipEventRecord(ev[1], e->stream));
-   Launch<P>::force(e->s, e->k, e->planeStride, e->block[0], e->stream);
-   ENS_HIP(e, hipEventRecord(ev[2], e->stream));
-   Launch<P>::node(e->s, e->k, e->planeStride, 0.0, e->block[1], e->stream);
-   ENS_HIP(e, hipEventRecord(ev[3], e->stream));
-   Launch<P>::kin(e->s, e->k, 0.0, e->block[2], e->stream);
+   if (e->brick) {
+      Launch<P>::nodal_brick_members(e->s, e->k, e->nodalBlocks, e->stream);
+      swap_nodes(e);
+      ENS_HIP(e, hipEventRecord(ev[2], e->stream));
+      ENS_HIP(e, hipEventRecord(ev[3], e->stream));
+      Launch<P>::kin_brick_members(e->s, e->k, e->kinBlocks, e->stream);
+   } else {
+      Launch<P>::force(e->s, e->k, e->planeStride, e->block[0], e->stream);
+      ENS_HIP(e, hipEventRecord(ev[2], e->stream));
+      Launch<P>::node(e->s, e->k, e->planeStride, 0.0, e->block[1], e->stream);
+      ENS_HIP(e, hipEventRecord(ev[3], e->stream));
+      Launch<P>::kin(e->s, e->k, 0.0, e->block[2], e->stream);
+   }
    ENS_HIP(e, hipEventRecord(ev[4], e->stream));
    Launch<P>::eos(e->s, e->k, e->stream);
    ENS_HIP(e, hipEventRecord(ev[5], e->stream));
@@ -169,9 +212,13 @@ int timed_cycle(lgpu_ensemble *e, int maxCycle, float ms[kENumKernels])
    ENS_HIP(e, hipEventRecord(ev[6], e->stream));
    ENS_HIP(e, hipGetLastError());
    ENS_HIP(e, hipStreamSynchronize(e->stream));
-   // ev[i]..ev[i+1] brackets: 0 time, 1 force, 2 node, 3 kin, 4 eos, 5 dt
-   const int order[6] = {kETime, kEForce, kENode, kEKin, kEEos, kEDt};
+   // ev[i]..ev[i+1] brackets: 0 time, 1 force (brick: nodal), 2 node (brick:
+   // empty), 3 kin, 4 eos, 5 dt
+   const int order[6] = {kETime, e->brick ? kENodalBrick : kEForce, kENode,
+                         e->brick ? kEKinBrick : kEKin, kEEos, kEDt};
+   for (int i = 0; i < kENumKernels; ++i) ms[i] = 0.f;
    for (int i = 0; i < 6; ++i) ENS_HIP(e, hipEventElapsedTime(&ms[order[i]], ev[i], ev[i + 1]));
+   if (e->brick) ms[kENode] = 0.f;
    for (auto &x : ev) (void)hipEventDestroy(x);
    return LGPU_OK;
 }
@@ -189,22 +236,101 @@ void drop_graph(lgpu_ensemble *e)
 {
    if (e->exec) (void)hipGraphExecDestroy(e->exec);
    if (e->graph) (void)hipGraphDestroy(e->graph);
-   e->exec = nullptr;
-   e->graph = nullptr;
+   if (e->exec2) (void)hipGraphExecDestroy(e->exec2);
+   if (e->graph2) (void)hipGraphDestroy(e->graph2);
+   e->exec = e->exec2 = nullptr;
+   e->graph = e->graph2 = nullptr;
    e->graphMaxCycle = -1;
 }
 
+int capture_cycle(lgpu_ensemble *e, int maxCycle, hipGraph_t &g, hipGraphExec_t &x)
+{
+   ENS_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
+   enqueue_cycle(e, maxCycle);
+   const hipError_t endErr = hipStreamEndCapture(e->stream, &g);
+   ENS_HIP(e, endErr);
+   ENS_HIP(e, hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
+   return LGPU_OK;
+}
+
+// One cycle graph; on the brick path a second one for the other node-buffer
+// parity (exec: captured at parity 0, exec2: at parity 1).  Capturing
+// advances the host's parity; it is put back afterwards.
 int build_graph(lgpu_ensemble *e, int maxCycle)
 {
    drop_graph(e);
-   ENS_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-   enqueue_cycle(e, maxCycle);
-   hipGraph_t g = nullptr;
-   const hipError_t endErr = hipStreamEndCapture(e->stream, &g);
-   ENS_HIP(e, endErr);
-   e->graph = g;
-   ENS_HIP(e, hipGraphInstantiate(&e->exec, g, nullptr, nullptr, 0));
+   if (e->brick) {
+      const int p0 = e->parity;
+      if (p0) swap_nodes(e);
+      int r = capture_cycle(e, maxCycle, e->graph, e->exec);
+      if (!r) r = capture_cycle(e, maxCycle, e->graph2, e->exec2);
+      if (e->parity != p0) swap_nodes(e);
+      if (r) return r;
+   } else {
+      const int r = capture_cycle(e, maxCycle, e->graph, e->exec);
+      if (r) return r;
+   }
    e->graphMaxCycle = maxCycle;
+   return LGPU_OK;
+}
+
+// Replays one cycle: the graph of the current parity (the host mirrors the
+// pointer swap the graph's nodal launch implies).
+int launch_graph(lgpu_ensemble *e)
+{
+   if (e->brick) {
+      ENS_HIP(e, hipGraphLaunch(e->parity ? e->exec2 : e->exec, e->stream));
+      swap_nodes(e);
+   } else {
+      ENS_HIP(e, hipGraphLaunch(e->exec, e->stream));
+   }
+   return LGPU_OK;
+}
+
+// Member tables of the brick path for the current chunk choices: per-member
+// bricks with offsets and first workgroups, and workgroup -> member maps.
+int build_brick_tables(lgpu_ensemble *e)
+{
+   const int M = e->nMembers;
+   std::vector<MemberBrick> mb((size_t)M);
+   std::vector<int32_t> nodalOf, kinOf;
+   int64_t nb = 0, kb = 0;
+   for (int m = 0; m < M; ++m) {
+      Brick g{};
+      init_brick(g, e->memberSize[(size_t)m]);
+      set_brick_chunk(g, e->nodalChunk);
+      set_kin_chunk(g, e->kinChunk);
+      mb[(size_t)m].g = g;
+      mb[(size_t)m].nodeOff = e->nodeStart[(size_t)m];
+      mb[(size_t)m].elemOff = e->elemStart[(size_t)m];
+      mb[(size_t)m].nodalBlock0 = (int32_t)nb;
+      mb[(size_t)m].kinBlock0 = (int32_t)kb;
+      const int64_t n1 = nodal_blocks(g), k1 = kin_blocks(g);
+      nodalOf.insert(nodalOf.end(), (size_t)n1, m);
+      kinOf.insert(kinOf.end(), (size_t)k1, m);
+      nb += n1;
+      kb += k1;
+   }
+   const size_t b0 = align_up(sizeof(MemberBrick) * (size_t)M);
+   const size_t b1 = align_up(4 * (size_t)nb), b2 = align_up(4 * (size_t)kb);
+   const size_t need = b0 + b1 + b2;
+   ENS_HIP(e, hipStreamSynchronize(e->stream));
+   if (need > e->tablesBytes) {
+      if (e->tables) ENS_HIP(e, hipFree(e->tables));
+      e->tables = nullptr;
+      e->tablesBytes = 0;
+      ENS_HIP(e, hipMalloc((void **)&e->tables, need));
+      e->tablesBytes = need;
+   }
+   ENS_HIP(e, hipMemcpy(e->tables, mb.data(), sizeof(MemberBrick) * (size_t)M, hipMemcpyHostToDevice));
+   ENS_HIP(e, hipMemcpy(e->tables + b0, nodalOf.data(), 4 * (size_t)nb, hipMemcpyHostToDevice));
+   ENS_HIP(e, hipMemcpy(e->tables + b0 + b1, kinOf.data(), 4 * (size_t)kb, hipMemcpyHostToDevice));
+   e->s.memberBrick = reinterpret_cast<const MemberBrick *>(e->tables);
+   e->s.nodalBlockMember = reinterpret_cast<const int32_t *>(e->tables + b0);
+   e->s.kinBlockMember = reinterpret_cast<const int32_t *>(e->tables + b0 + b1);
+   e->nodalBlocks = (unsigned)nb;
+   e->kinBlocks = (unsigned)kb;
+   drop_graph(e);
    return LGPU_OK;
 }
 
@@ -304,6 +430,11 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
    };
    e->nMembers = nMembers;
    e->precision = prec;
+   // Brick path when every member is the reference's brick (and the generic
+   // kernels are not asked for).
+   e->brick = !(members[0].flags & LGPU_FLAG_GENERIC);
+   for (int32_t m = 0; m < nMembers && e->brick; ++m) e->brick = is_reference_brick(&members[m]);
+   for (int32_t m = 0; m < nMembers; ++m) e->memberSize.push_back(members[m].sizeX);
    const lgpu_constants &kk = members[0].k;
    e->k = Consts{kk.e_cut, kk.p_cut, kk.q_cut, kk.v_cut, kk.u_cut, kk.hgcoef, kk.ss4o3,
                  kk.qstop, kk.monoq_max_slope, kk.monoq_limiter_mult, kk.qlc_monoq,
@@ -342,6 +473,11 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
    auto add = [&](void **dst, size_t bytes) { slots.push_back({dst, align_up(bytes)}); };
    double **stateNode[] = {&s.x, &s.y, &s.z, &s.xd, &s.yd, &s.zd};
    for (double **p : stateNode) add((void **)p, 8 * N);
+   if (e->brick) {
+      // the second node buffer is state too: either may be current
+      double **second[] = {&s.xo, &s.yo, &s.zo, &s.xdo, &s.ydo, &s.zdo};
+      for (double **p : second) add((void **)p, 8 * N);
+   }
    double **stateElem[] = {&s.e, &s.v};
    for (double **p : stateElem) add((void **)p, 8 * E);
    void **statePqs[] = {&s.p, &s.q, &s.ss};
@@ -368,7 +504,7 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
    add((void **)&s.nodelist, 32 * E);
    const int32_t **nbr[] = {&s.lxim, &s.lxip, &s.letam, &s.letap, &s.lzetam, &s.lzetap, &s.elemBC};
    for (const int32_t **p : nbr) add((void **)p, 4 * E);
-   add(&s.corner, cb * (size_t)plane * 48);
+   if (!e->brick) add(&s.corner, cb * (size_t)plane * 48);
    add((void **)&s.perm, 4 * nPerm);
    add((void **)&s.regStart, 4 * ((size_t)numRegTot + 1));
    add((void **)&s.regRep, 4 * (size_t)numRegTot);
@@ -469,6 +605,7 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
              h2d((void *)s.memberBlockStart, blockStart.data(), 4 * blockStart.size()) &&
              h2d(s.v, ones.data(), 8 * E);
    if (!ok) return bail(LGPU_EDEVICE);
+   if (e->brick && build_brick_tables(e) != LGPU_OK) return bail(LGPU_EDEVICE);
    if (hipHostMalloc((void **)&e->host, (size_t)nMembers * (9 * 8 + 8) + 64 +
                                             (size_t)nMembers * kNumErrSites * 4,
                      hipHostMallocDefault) != hipSuccess)
@@ -483,6 +620,7 @@ void lgpu_ensemble_destroy(lgpu_ensemble *e)
    if (e->stream) (void)hipStreamSynchronize(e->stream);
    drop_graph(e);
    if (e->arena) (void)hipFree(e->arena);
+   if (e->tables) (void)hipFree(e->tables);
    if (e->host) (void)hipHostFree(e->host);
    if (e->stream) (void)hipStreamDestroy(e->stream);
    delete e;
@@ -584,7 +722,8 @@ int lgpu_ensemble_run(lgpu_ensemble *e, lgpu_time_state *ts, int32_t maxCycle, i
                e->kMs[i] += ms[i];
             }
          } else if (e->useGraph) {
-            ENS_HIP(e, hipGraphLaunch(e->exec, e->stream));
+            const int r = launch_graph(e);
+            if (r) return r;
          } else {
             enqueue_cycle(e, maxCycle);
             ENS_HIP(e, hipGetLastError());
@@ -649,6 +788,11 @@ int lgpu_ensemble_set_option(lgpu_ensemble *e, int option, int64_t value)
       e->s.xcdRemap = value ? 1 : 0;
       drop_graph(e);
       return LGPU_OK;
+   case LGPU_ENS_OPT_BRICK_CHUNK:
+   case LGPU_ENS_OPT_KIN_CHUNK:
+      if (!e->brick || value < 1 || value > 1024) return LGPU_EINVAL;
+      (option == LGPU_ENS_OPT_BRICK_CHUNK ? e->nodalChunk : e->kinChunk) = (int)value;
+      return build_brick_tables(e);
    default:
       return LGPU_EINVAL;
    }
@@ -664,6 +808,15 @@ int lgpu_ensemble_get_option(lgpu_ensemble *e, int option, int64_t *value)
    case LGPU_ENS_OPT_BLOCK_NODE:
    case LGPU_ENS_OPT_BLOCK_KIN: *value = e->block[option - LGPU_ENS_OPT_BLOCK_FORCE]; return LGPU_OK;
    case LGPU_ENS_OPT_XCD_REMAP: *value = e->s.xcdRemap; return LGPU_OK;
+   case LGPU_ENS_OPT_BRICK_CHUNK:
+      if (!e->brick) return LGPU_EINVAL;
+      *value = e->nodalChunk;
+      return LGPU_OK;
+   case LGPU_ENS_OPT_KIN_CHUNK:
+      if (!e->brick) return LGPU_EINVAL;
+      *value = e->kinChunk;
+      return LGPU_OK;
+   case LGPU_ENS_OPT_PATH: *value = e->brick ? 1 : 0; return LGPU_OK;
    default: return LGPU_EINVAL;
    }
 }
@@ -671,12 +824,14 @@ int lgpu_ensemble_get_option(lgpu_ensemble *e, int option, int64_t *value)
 int lgpu_ensemble_autotune(lgpu_ensemble *e, const lgpu_time_state *ts, int32_t trialCycles)
 {
    if (!e || !ts || trialCycles < 1) return LGPU_EINVAL;
-   // The trials advance the state: snapshot the state part of the arena and
-   // restore it after every trial.
+   // The trials advance the state: snapshot the state part of the arena (on
+   // the brick path both node buffers) and restore it, and the node-buffer
+   // parity, after every trial.
    char *snap = nullptr;
    ENS_HIP(e, hipMalloc((void **)&snap, e->stateBytes));
    ENS_HIP(e, hipMemcpyAsync(snap, e->arena, e->stateBytes, hipMemcpyDeviceToDevice, e->stream));
    const int M = e->nMembers;
+   const int parity0 = e->parity;
    std::vector<lgpu_time_state> tsv(ts, ts + M);
    const bool savedTiming = e->timing, savedGraph = e->useGraph;
    e->timing = true;
@@ -693,27 +848,63 @@ int lgpu_ensemble_autotune(lgpu_ensemble *e, const lgpu_time_state *ts, int32_t 
          ms[i] = e->kLaunches[i] ? (float)(e->kMs[i] / e->kLaunches[i]) : 0.f;
       ENS_HIP(e, hipMemcpyAsync(e->arena, snap, e->stateBytes, hipMemcpyDeviceToDevice, e->stream));
       ENS_HIP(e, hipStreamSynchronize(e->stream));
+      if (e->parity != parity0) swap_nodes(e);
       return LGPU_OK;
    };
-   // Each of the three launch-shape-sensitive kernels is tuned over the block
-   // sizes, then the XCD mapping over the tuned shapes.
-   const int cands[3] = {64, 128, 256};
-   const int kern[3] = {kEForce, kENode, kEKin};
    int r = LGPU_OK;
-   for (int which = 0; which < 3 && !r; ++which) {
+   if (e->brick) {
+      // Chunk of each fused kernel over a few candidates, then the XCD mapping.
+      const int nodalCands[4] = {4, 8, 13, 22}, kinCands[3] = {4, 8, 16};
       float best = 1e30f;
-      int bestB = e->block[which];
-      for (int b : cands) {
-         e->block[which] = b;
+      int bestC = e->nodalChunk;
+      for (int c : nodalCands) {
+         e->nodalChunk = c;
+         r = build_brick_tables(e);
          float ms[kENumKernels];
-         r = trial(ms);
+         if (!r) r = trial(ms);
          if (r) break;
-         if (ms[kern[which]] < best) {
-            best = ms[kern[which]];
-            bestB = b;
+         if (ms[kENodalBrick] < best) {
+            best = ms[kENodalBrick];
+            bestC = c;
          }
       }
-      e->block[which] = bestB;
+      e->nodalChunk = bestC;
+      best = 1e30f;
+      bestC = e->kinChunk;
+      for (int c : kinCands) {
+         if (r) break;
+         e->kinChunk = c;
+         r = build_brick_tables(e);
+         float ms[kENumKernels];
+         if (!r) r = trial(ms);
+         if (r) break;
+         if (ms[kEKinBrick] < best) {
+            best = ms[kEKinBrick];
+            bestC = c;
+         }
+      }
+      e->kinChunk = bestC;
+      if (!r) r = build_brick_tables(e);
+   } else {
+      // Each of the three launch-shape-sensitive kernels is tuned over the
+      // block sizes.
+      const int cands[3] = {64, 128, 256};
+      const int kern[3] = {kEForce, kENode, kEKin};
+      for (int which = 0; which < 3 && !r; ++which) {
+         float best = 1e30f;
+         int bestB = e->block[which];
+         for (int b : cands) {
+            e->block[which] = b;
+            float ms[kENumKernels];
+            r = trial(ms);
+            if (r) break;
+            if (ms[kern[which]] < best) {
+               best = ms[kern[which]];
+               bestB = b;
+            }
+         }
+         e->block[which] = bestB;
+      }
    }
    if (!r) {
       float tot[2];
@@ -721,7 +912,9 @@ int lgpu_ensemble_autotune(lgpu_ensemble *e, const lgpu_time_state *ts, int32_t 
          e->s.xcdRemap = remap;
          float ms[kENumKernels];
          r = trial(ms);
-         tot[remap] = ms[kEForce] + ms[kENode] + ms[kEKin] + ms[kEEos];
+         tot[remap] = 0.f;
+         for (int i = 0; i < kENumKernels; ++i)
+            if (i != kETime && i != kEDt) tot[remap] += ms[i];
       }
       if (!r) e->s.xcdRemap = (tot[1] <= tot[0]) ? 1 : 0;
    }

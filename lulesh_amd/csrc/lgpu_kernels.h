@@ -92,6 +92,14 @@ struct Brick {
    int32_t kinTiles, kinChunk, kinChunks;
 };
 
+// One ensemble member on the brick path: its brick, where its nodes and
+// elements start in the concatenated arrays, and its first workgroup in the
+// k_nodal_brick and k_kin_brick grids.
+struct MemberBrick {
+   Brick g;
+   int32_t nodeOff, elemOff, nodalBlock0, kinBlock0;
+};
+
 // Everything a cycle touches on the device.
 struct DevState {
    int32_t numElem, numNode, numReg;
@@ -139,6 +147,10 @@ struct DevState {
    const int32_t *blockMember;     // k_eos block -> member (-1: padding)
    const int32_t *memberBlockStart; // nMembers+1: k_eos blocks of each member
    double *memberDtOut;            // 2 per member: dtcourant, dthydro
+   // ensemble on the brick path (memberBrick non-null)
+   const MemberBrick *memberBrick;
+   const int32_t *nodalBlockMember; // k_nodal_brick workgroup -> member
+   const int32_t *kinBlockMember;   // k_kin_brick workgroup -> member
 };
 
 } // namespace lgpu

@@ -28,6 +28,11 @@ struct Launch {
    static int nodal_brick_blocks_per_cu();
    static void kin(const DevState &s, const Consts &k, double dt, int block, hipStream_t st);
    static void kin_brick(const DevState &s, const Consts &k, double dt, hipStream_t st);
+   // ensemble on the brick path: one launch over every member's workgroups
+   // (16x16-thread tiles), the grid sizes from the member tables
+   static void nodal_brick_members(const DevState &s, const Consts &k, unsigned grid,
+                                   hipStream_t st);
+   static void kin_brick_members(const DevState &s, const Consts &k, unsigned grid, hipStream_t st);
    static void eos(const DevState &s, const Consts &k, hipStream_t st);
    static void dt_reduce(const DevState &s, hipStream_t st);
 };

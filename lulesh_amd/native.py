@@ -187,6 +187,8 @@ def host_lib() -> ctypes.CDLL:
         lib.lh_ensemble_draw.restype = None
         lib.lh_ensemble_create.argtypes = [i32, _i32p, _f64p, _i32p, _i32p, _i32p, i32]
         lib.lh_ensemble_create.restype = vp
+        lib.lh_ensemble_create_flags.argtypes = [i32, _i32p, _f64p, _i32p, _i32p, _i32p, i32, i32]
+        lib.lh_ensemble_create_flags.restype = vp
         lib.lh_ensemble_destroy.argtypes = [vp]
         lib.lh_ensemble_destroy.restype = None
         lib.lh_ensemble_run.argtypes = [vp, i32, _i32p]
@@ -405,13 +407,14 @@ def ensemble_draw(n: int, seed: int = 1234):
 
 
 ENS_OPT_GRAPH, ENS_OPT_TIMING, ENS_OPT_BLOCK_FORCE, ENS_OPT_BLOCK_NODE, ENS_OPT_BLOCK_KIN, \
-    ENS_OPT_XCD_REMAP = 1, 2, 3, 4, 5, 6
+    ENS_OPT_XCD_REMAP, ENS_OPT_BRICK_CHUNK, ENS_OPT_KIN_CHUNK, ENS_OPT_PATH = 1, 2, 3, 4, 5, 6, 7, 8, 9
 
 
 class Ensemble:
     """Host Domains advanced together on the device (lh_ensemble / lgpu_ensemble)."""
 
-    def __init__(self, sizes, emults, balances, costs, regions=11, precision: int = FP64):
+    def __init__(self, sizes, emults, balances, costs, regions=11, precision: int = FP64,
+                 flags: int = 0):
         lib = host_lib()
         n = len(sizes)
         self._arrays = [np.ascontiguousarray(sizes, np.int32),
@@ -421,9 +424,10 @@ class Ensemble:
                         np.ascontiguousarray(balances, np.int32),
                         np.ascontiguousarray(costs, np.int32)]
         sz, em, nr, b, c = self._arrays
-        self.ptr = lib.lh_ensemble_create(n, sz.ctypes.data_as(_i32p), em.ctypes.data_as(_f64p),
-                                          nr.ctypes.data_as(_i32p), b.ctypes.data_as(_i32p),
-                                          c.ctypes.data_as(_i32p), precision)
+        self.ptr = lib.lh_ensemble_create_flags(n, sz.ctypes.data_as(_i32p),
+                                                em.ctypes.data_as(_f64p), nr.ctypes.data_as(_i32p),
+                                                b.ctypes.data_as(_i32p), c.ctypes.data_as(_i32p),
+                                                precision, flags)
         if not self.ptr:
             raise RuntimeError("lh_ensemble_create failed")
         self.n = n
@@ -471,10 +475,18 @@ class Ensemble:
     def autotune(self, trial_cycles: int = 3) -> dict:
         ts = self.time_states()
         check(gpu_lib().lgpu_ensemble_autotune(self.dev, ts, trial_cycles))
-        return {"block_force": self.get_option(ENS_OPT_BLOCK_FORCE),
+        if self.brick:
+            return {"path": "brick", "brick_chunk": self.get_option(ENS_OPT_BRICK_CHUNK),
+                    "kin_chunk": self.get_option(ENS_OPT_KIN_CHUNK),
+                    "xcd_remap": self.get_option(ENS_OPT_XCD_REMAP)}
+        return {"path": "generic", "block_force": self.get_option(ENS_OPT_BLOCK_FORCE),
                 "block_node": self.get_option(ENS_OPT_BLOCK_NODE),
                 "block_kin": self.get_option(ENS_OPT_BLOCK_KIN),
                 "xcd_remap": self.get_option(ENS_OPT_XCD_REMAP)}
+
+    @property
+    def brick(self) -> bool:
+        return self.get_option(ENS_OPT_PATH) == 1
 
     def kernel_stats(self) -> dict:
         lib = gpu_lib()

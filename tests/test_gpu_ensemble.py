@@ -27,8 +27,13 @@ def _single(s, b, c, m, its, precision=la.FP64):
 
 
 @pytest.mark.parametrize("graph", [True, False])
-def test_members_equal_single_runs(graph):
-    ens = la.Ensemble(**SMALL)
+@pytest.mark.parametrize("generic", [False, True], ids=["brick", "generic"])
+def test_members_equal_single_runs(graph, generic):
+    """Brick path (fused kernels over every member's workgroups, node buffers
+    ping-ponging per cycle) and CSR-gather path, with and without the cycle
+    graphs: every member equals its own single-problem run bit for bit."""
+    ens = la.Ensemble(**SMALL, flags=la.native.FLAG_GENERIC if generic else 0)
+    assert ens.brick == (not generic)
     ens.set_option(la.native.ENS_OPT_GRAPH, int(graph))
     st, status = ens.run(40)
     assert st == 0 and not status.any()
@@ -88,11 +93,17 @@ def test_mixed_ensemble_within_bound():
         assert err <= 1e-6
 
 
-def test_autotune_keeps_state():
-    ens = la.Ensemble(**SMALL)
+@pytest.mark.parametrize("generic", [False, True], ids=["brick", "generic"])
+def test_autotune_keeps_state(generic):
+    """The autotuner's trial cycles are undone (state and, on the brick path,
+    the node-buffer parity): the run after it equals a single run."""
+    ens = la.Ensemble(**SMALL, flags=la.native.FLAG_GENERIC if generic else 0)
     before = [m.get("cycle") for m in ens.members]
-    choice = ens.autotune(trial_cycles=2)
-    assert choice["block_force"] in (64, 128, 256)
+    choice = ens.autotune(trial_cycles=3)
+    if generic:
+        assert choice["block_force"] in (64, 128, 256)
+    else:
+        assert choice["brick_chunk"] in (4, 8, 13, 22) and choice["kin_chunk"] in (4, 8, 16)
     st, _ = ens.run(10)
     assert st == 0
     ens.sync()
@@ -101,3 +112,20 @@ def test_autotune_keeps_state():
     assert before[3] == 0 and mem.get("cycle") == 10
     for f in STATE_FIELDS:
         assert np.array_equal(mem.real(f), ref.real(f)), f
+
+
+def test_brick_chunks_do_not_change_results():
+    """Member results do not depend on the brick kernels' z-chunking."""
+    runs = []
+    for nodal, kin in ((8, 8), (3, 5)):
+        ens = la.Ensemble(**SMALL)
+        ens.set_option(la.native.ENS_OPT_BRICK_CHUNK, nodal)
+        ens.set_option(la.native.ENS_OPT_KIN_CHUNK, kin)
+        st, _ = ens.run(17)
+        assert st == 0
+        ens.sync()
+        runs.append([{f: m.real(f).copy() for f in STATE_FIELDS} for m in ens.members])
+        ens.close()
+    for a, b in zip(*runs):
+        for f in STATE_FIELDS:
+            assert np.array_equal(a[f], b[f]), f
