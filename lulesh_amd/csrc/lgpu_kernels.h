@@ -48,7 +48,6 @@ struct PolicyFp64 {
    using grad_t = double;   // Q gradients + limiter arithmetic and storage
    using eos_t = double;    // EOS pressure / sound-speed arithmetic
    using nodesum_t = double; // per-node sums of the corner forces
-   using kin_t = double;     // characteristic length and strain rate
    using pqs_t = double;     // stored p, q, ss, elemMass, arealg, vdov
    static constexpr bool kLocalCoords = false;
    // brick kernel: load the node update's operands before the element work
@@ -66,7 +65,6 @@ struct PolicyMixed {
    using grad_t = float;
    using eos_t = float;
    using nodesum_t = float;
-   using kin_t = double;
    using pqs_t = float;
    static constexpr bool kLocalCoords = true;
    static constexpr bool kPrefetchNode = true;
