@@ -168,9 +168,13 @@ enum lgpu_option {
                                      (default: all)                          */
    LGPU_OPT_KIN_CHUNK = 7,        /* element planes per workgroup of the
                                      brick kinematics kernel (1..mesh edge)  */
-   LGPU_OPT_BRICK_ROWS = 8        /* thread rows of a fused brick workgroup:
+   LGPU_OPT_BRICK_ROWS = 8,       /* thread rows of a fused brick workgroup:
                                      16 (16x16 element columns, default) or 8
                                      (16x8)                                  */
+   LGPU_OPT_EOS_INTERLEAVE = 9    /* k_eos dispatch order: 0 region order,
+                                     heaviest regions first; k > 0 (default
+                                     4) one workgroup of the largest EOS
+                                     repeat count every k+1 dispatches      */
 };
 int lgpu_set_option(lgpu_ctx *ctx, int option, int64_t value);
 /* Current value of an option (the brick chunks report the defaults chosen at

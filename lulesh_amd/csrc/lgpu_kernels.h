@@ -134,6 +134,9 @@ struct DevState {
    Brick brick;             // valid when isBrick
    int32_t isBrick;
    int32_t brickFaces; // k_eos computes face neighbours and BC masks from the brick
+   // k_eos dispatch order: workgroup i processes perm block eosBlockOrder[i]
+   // (null: block i)
+   const int32_t *eosBlockOrder;
    // Ensemble of independent problems (nMembers > 0): members are
    // concatenated; each has its own dt, loop flag and error flags.
    int32_t nMembers;

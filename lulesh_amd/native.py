@@ -24,7 +24,7 @@ FIELDS = [
 NODE_FIELDS = set(FIELDS[:13])
 FP64, MIXED, FP64_FMA = 0, 1, 2
 OPT_KEEP_DIAGNOSTICS, OPT_KERNEL_TIMING, OPT_BLOCK_SIZE, OPT_BRICK_CHUNK, OPT_XCD_REMAP = 1, 2, 3, 4, 5
-OPT_KERNEL_TIMING_MASK, OPT_KIN_CHUNK, OPT_BRICK_ROWS = 6, 7, 8
+OPT_KERNEL_TIMING_MASK, OPT_KIN_CHUNK, OPT_BRICK_ROWS, OPT_EOS_INTERLEAVE = 6, 7, 8, 9
 FLAG_GENERIC = 1
 STATUS = {0: "ok", -1: "VolumeError", -2: "QStopError", -100: "EINVAL", -101: "EDEVICE",
           -102: "ENOMEM"}

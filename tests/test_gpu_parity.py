@@ -196,6 +196,32 @@ def test_region_layout_does_not_change_results():
         assert np.array_equal(a.real(f), b.real(f)), f
 
 
+def test_eos_dispatch_order_does_not_change_results():
+    """k_eos may visit its workgroups in any order (each element's EOS is
+    independent, the dt minimum is order-free): 0 (region order), the
+    default and other interleaves give identical states and dt."""
+    runs = []
+    for k in (0, 1, 4, 7):
+        o = OracleDomain(20, 21, 2, 2)
+        ctx = context_from_oracle(o)
+        ctx.set_option(la.native.OPT_EOS_INTERLEAVE, k)
+        assert ctx.get_option(la.native.OPT_EOS_INTERLEAVE) == k
+        upload_state(ctx, o)
+        dt, dts = o.get("deltatime"), []
+        for _ in range(10):
+            assert ctx.lagrange_nodal(dt) == 0
+            assert ctx.lagrange_elements(dt) == 0
+            st, c, h = ctx.time_constraints()
+            assert st == 0
+            dts.append((c, h))
+            dt = min(c / 2, dt * 1.1)
+        runs.append(([ctx.download(f) for f in STATE_FIELDS], dts))
+    for fields, dts in runs[1:]:
+        assert dts == runs[0][1]
+        for a, b in zip(fields, runs[0][0]):
+            assert np.array_equal(a, b)
+
+
 def test_qstop_error_code():
     o = OracleDomain(6)
     o.call("time_increment")
