@@ -360,7 +360,10 @@ def main() -> None:
         cpu = cpu_baseline(a.size, a.regions, a.balance, a.cost, a.cpu_seconds)
 
     line = {
-        "metric": "FOM zones/us @s=192 FP64 (LULESH Sedov, LagrangeLeapFrog cycle)",
+        # the headline metric; value is its first part (FP64 FOM), the mixed
+        # form, the HBM fraction (roofline) and the ensemble are reported in
+        # their own fields of this line
+        "metric": "FOM zones/\u00b5s @s=192 FP64 vs mixed; HBM GB/s % of peak; ensemble zones/\u00b5s",
         "value": round(fom, 2),
         "unit": "zones/us",
         "n_gpus": 1,

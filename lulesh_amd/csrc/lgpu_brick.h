@@ -120,7 +120,7 @@ inline void set_kin_chunk(Brick &g, int chunk)
 }
 
 // Tiling of an s^3 brick with the default chunks (the caller may pick
-// others): 15x15-node tiles of the fused nodal kernel, 15x15-element tiles
+// others): 15x15-node tiles of the fused nodal kernel, 16x16-element tiles
 // of the kinematics kernel.
 inline void init_brick(Brick &g, int s)
 {
@@ -129,10 +129,10 @@ inline void init_brick(Brick &g, int s)
    g.tilesX = g.tilesY = (g.en + 14) / 15;
    g.divS = make_fastdiv((uint32_t)g.s);
    g.divS2 = make_fastdiv((uint32_t)(g.s * g.s));
-   g.kinTiles = (g.s + 14) / 15;
+   g.kinTiles = (g.s + 15) / 16;
    g.rows = 16;
    set_brick_chunk(g, 8);
-   set_kin_chunk(g, 8);
+   set_kin_chunk(g, 12);
 }
 
 // Workgroups of one launch of each brick kernel.

@@ -116,7 +116,7 @@ struct lgpu_ensemble {
    hipGraph_t graph2 = nullptr;
    hipGraphExec_t exec2 = nullptr;
    std::vector<int32_t> memberSize;
-   int nodalChunk = 8, kinChunk = 8;
+   int nodalChunk = 8, kinChunk = 12;
    unsigned nodalBlocks = 0, kinBlocks = 0;
    char *tables = nullptr;
    size_t tablesBytes = 0;
@@ -854,7 +854,7 @@ int lgpu_ensemble_autotune(lgpu_ensemble *e, const lgpu_time_state *ts, int32_t 
    int r = LGPU_OK;
    if (e->brick) {
       // Chunk of each fused kernel over a few candidates, then the XCD mapping.
-      const int nodalCands[4] = {4, 8, 13, 22}, kinCands[3] = {4, 8, 16};
+      const int nodalCands[4] = {4, 8, 13, 22}, kinCands[4] = {4, 8, 12, 16};
       float best = 1e30f;
       int bestC = e->nodalChunk;
       for (int c : nodalCands) {

@@ -103,7 +103,7 @@ def test_autotune_keeps_state(generic):
     if generic:
         assert choice["block_force"] in (64, 128, 256)
     else:
-        assert choice["brick_chunk"] in (4, 8, 13, 22) and choice["kin_chunk"] in (4, 8, 16)
+        assert choice["brick_chunk"] in (4, 8, 13, 22) and choice["kin_chunk"] in (4, 8, 12, 16)
     st, _ = ens.run(10)
     assert st == 0
     ens.sync()
