@@ -10,6 +10,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <string>
 #include <vector>
@@ -204,21 +205,6 @@ int32_t region_rep(int32_t r, int32_t numReg, int32_t cost)
    if (r < numReg / 2) return 1;
    if (r < numReg - (numReg + 15) / 20) return 1 + cost;
    return 10 * (1 + cost);
-}
-
-// Order in which the regions are laid out for k_eos: largest EOS repeat
-// count first (stable within a count), so that the longest-running
-// workgroups are dispatched first and the short ones fill in behind them.
-// An element's EOS does not depend on where it sits in the list, and the dt
-// minimum does not depend on the order of its candidates.
-std::vector<int32_t> heavy_first(int32_t numReg, int32_t cost)
-{
-   std::vector<int32_t> order((size_t)numReg);
-   for (int32_t r = 0; r < numReg; ++r) order[(size_t)r] = r;
-   std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-      return region_rep(a, numReg, cost) > region_rep(b, numReg, cost);
-   });
-   return order;
 }
 
 // Device array of a field.  `narrow`: stored as FP32 in the mixed form (the
@@ -543,32 +529,47 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
    const int32_t ns[3] = {d->numSymmX, d->numSymmY, d->numSymmZ};
    for (int a = 0; a < 3; ++a)
       for (int32_t i = 0; i < ns[a]; ++i) bc[(size_t)sy[a][i]] |= (uint8_t)(1u << a);
+   // The EOS work list: elements grouped by their region's EOS repeat count,
+   // largest first, and in mesh order within a group.  An element's EOS
+   // depends only on the element and its repeat count (lulesh.cc:2387-2401
+   // runs the same code for every region), so merging the regions of equal
+   // count into one ascending list is exact; it turns the regions' short runs
+   // (~38 elements on average) into long ones, so k_eos reads whole cache
+   // lines.  regStart/regRep then describe the groups (k_eos finds an
+   // element's count by a binary search over them).
    std::vector<int32_t> perm;
    perm.reserve(E);
-   std::vector<int32_t> regStart((size_t)d->numReg + 1, 0), regRep((size_t)d->numReg, 1);
-   const std::vector<int32_t> order = heavy_first(d->numReg, d->cost);
-   for (int i = 0; i < d->numReg; ++i) {
-      const int32_t r = order[(size_t)i];
-      regStart[(size_t)i] = (int32_t)perm.size();
-      perm.insert(perm.end(), d->regElemlist[r], d->regElemlist[r] + d->regElemSize[r]);
-      regRep[(size_t)i] = region_rep(r, d->numReg, d->cost);
+   std::vector<int32_t> repOf(E, 1);
+   std::vector<int32_t> reps;
+   for (int r = 0; r < d->numReg; ++r) {
+      const int32_t rep = region_rep(r, d->numReg, d->cost);
+      if (d->regElemSize[r] > 0 && std::find(reps.begin(), reps.end(), rep) == reps.end())
+         reps.push_back(rep);
+      for (int32_t i = 0; i < d->regElemSize[r]; ++i) repOf[(size_t)d->regElemlist[r][i]] = rep;
    }
-   regStart[(size_t)d->numReg] = (int32_t)perm.size();
-   // Repeat count of each k_eos block: the largest over the regions it holds.
+   std::sort(reps.begin(), reps.end(), std::greater<int32_t>());
+   const int numGroups = std::max<int>(1, (int)reps.size());
+   std::vector<int32_t> regStart((size_t)numGroups + 1, 0), regRep((size_t)numGroups, 1);
+   for (size_t k = 0; k < reps.size(); ++k) {
+      regStart[k] = (int32_t)perm.size();
+      regRep[k] = reps[k];
+      for (size_t el = 0; el < E; ++el)
+         if (repOf[el] == reps[k]) perm.push_back((int32_t)el);
+   }
+   regStart[(size_t)numGroups] = (int32_t)perm.size();
+   s.numReg = numGroups;
+   // Repeat count of each k_eos block: the largest over the groups it holds.
    c->eosBlockRep.assign(eosBlocks, 1);
-   for (int i = 0; i < d->numReg; ++i)
+   for (int i = 0; i < numGroups; ++i)
       for (int64_t p = regStart[(size_t)i]; p < regStart[(size_t)i + 1]; p += kEosBlock) {
          const size_t b = (size_t)(p / kEosBlock);
          c->eosBlockRep[b] = std::max(c->eosBlockRep[b], regRep[(size_t)i]);
       }
-   for (int i = 0; i < d->numReg; ++i)
+   for (int i = 0; i < numGroups; ++i)
       if (regStart[(size_t)i + 1] > regStart[(size_t)i]) {
          const size_t b = (size_t)((regStart[(size_t)i + 1] - 1) / kEosBlock);
          c->eosBlockRep[b] = std::max(c->eosBlockRep[b], regRep[(size_t)i]);
       }
-   // Empty regions sit at the same offset as the next one; the binary search
-   // in k_eos picks the last region starting at or before a position, which
-   // is the non-empty one holding it.
    std::vector<double> ones(E, 1.0);
 
    auto h2d = [&](const void *dst, const void *src, size_t bytes) {

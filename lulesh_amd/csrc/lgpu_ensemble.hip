@@ -12,6 +12,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <new>
 #include <string>
 #include <vector>
@@ -566,18 +567,23 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
       for (int a = 0; a < 3; ++a)
          for (int32_t i = 0; i < ns[a]; ++i) bc[(size_t)(n0 + sy[a][i])] |= (uint8_t)(1u << a);
       int64_t pos = (int64_t)blockStart[m] * kEosBlock;
-      // Largest EOS repeat count first within the member (as for a single
-      // domain): long workgroups start first, short ones fill in behind.
-      std::vector<int32_t> order((size_t)d.numReg);
-      for (int32_t r = 0; r < d.numReg; ++r) order[(size_t)r] = r;
-      std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t b) {
-         return region_rep(a, d.numReg, d.cost) > region_rep(b, d.numReg, d.cost);
-      });
-      for (int32_t i = 0; i < d.numReg; ++i) {
-         const int32_t r = order[(size_t)i];
-         regStart[(size_t)(regBase + i)] = (int32_t)pos;
-         regRep[(size_t)(regBase + i)] = region_rep(r, d.numReg, d.cost);
-         for (int32_t i = 0; i < d.regElemSize[r]; ++i) perm[(size_t)pos++] = d.regElemlist[r][i] + e0;
+      // As for a single domain: the member's elements grouped by EOS repeat
+      // count, largest first, in mesh order within a group; the member's
+      // remaining region slots stay empty at its end.
+      std::vector<int32_t> repOf((size_t)d.numElem, 1), reps;
+      for (int32_t r = 0; r < d.numReg; ++r) {
+         const int32_t rep = region_rep(r, d.numReg, d.cost);
+         if (d.regElemSize[r] > 0 && std::find(reps.begin(), reps.end(), rep) == reps.end())
+            reps.push_back(rep);
+         for (int32_t i = 0; i < d.regElemSize[r]; ++i) repOf[(size_t)d.regElemlist[r][i]] = rep;
+      }
+      std::sort(reps.begin(), reps.end(), std::greater<int32_t>());
+      for (int32_t k = 0; k < d.numReg; ++k) {
+         regStart[(size_t)(regBase + k)] = (int32_t)pos;
+         regRep[(size_t)(regBase + k)] = k < (int32_t)reps.size() ? reps[(size_t)k] : 1;
+         if (k < (int32_t)reps.size())
+            for (int32_t el = 0; el < d.numElem; ++el)
+               if (repOf[(size_t)el] == reps[(size_t)k]) perm[(size_t)pos++] = el + e0;
       }
       regBase += d.numReg;
       for (int32_t b = blockStart[m]; b < blockStart[m + 1]; ++b) blockMember[(size_t)b] = m;
