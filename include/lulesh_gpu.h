@@ -171,10 +171,12 @@ enum lgpu_option {
    LGPU_OPT_BRICK_ROWS = 8,       /* thread rows of a fused brick workgroup:
                                      16 (16x16 element columns, default) or 8
                                      (16x8)                                  */
-   LGPU_OPT_EOS_INTERLEAVE = 9    /* k_eos dispatch order: 0 region order,
-                                     heaviest regions first; k > 0 (default
-                                     4) one workgroup of the largest EOS
-                                     repeat count every k+1 dispatches      */
+   LGPU_OPT_EOS_INTERLEAVE = 9    /* k_eos dispatch order: 0 work-list order,
+                                     largest EOS repeat count first; k > 0
+                                     (default 4) one workgroup of the largest
+                                     count every k+1 dispatches (an odd k is
+                                     raised by one: the 8 XCDs are dealt
+                                     workgroups in turn)                    */
 };
 int lgpu_set_option(lgpu_ctx *ctx, int option, int64_t value);
 /* Current value of an option (the brick chunks report the defaults chosen at

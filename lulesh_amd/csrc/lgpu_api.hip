@@ -209,12 +209,16 @@ int32_t region_rep(int32_t r, int32_t numReg, int32_t cost)
 
 // Device array of a field.  `narrow`: stored as FP32 in the mixed form (the
 // Q gradients, p, q, ss, elemMass, arealg and vdov).
-// k_eos dispatch order.  k = 0: region order (heaviest regions first).
-// k > 0: the workgroups of the largest repeat count are spread out, one
-// every k + 1 dispatches, with the others (in region order) between them, so
-// that compute-bound and memory-bound workgroups share the CUs.
+// k_eos dispatch order.  k = 0: work-list order (largest repeat count
+// first).  k > 0: the workgroups of the largest repeat count are spread out,
+// one every k + 1 dispatches, with the others (in list order) between them,
+// so that compute-bound and memory-bound workgroups share the CUs.  The
+// dispatcher deals workgroups to the 8 XCDs in turn, so an even period would
+// put every heavy workgroup on the same few XCDs (measured: period 4 nearly
+// doubles the EOS time); an even k + 1 is raised to the next odd period.
 int set_eos_order(lgpu_ctx *c, int k)
 {
+   if (k > 0 && (k + 1) % 2 == 0) ++k;
    const size_t nb = c->eosBlockRep.size();
    std::vector<int32_t> order;
    order.reserve(nb);

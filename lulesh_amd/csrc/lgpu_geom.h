@@ -156,30 +156,39 @@ LG_DEV T voluder_form(const T (&a)[8], const T (&b)[8])
           (a[A2] + a[A5]) * (b[A3] + b[A5]) + (a[A3] + a[A5]) * (b[A2] + b[A5]);
 }
 
-template <typename T, int N, int A0, int A1, int A2, int A3, int A4, int A5>
+// kTwelfth = false leaves out the factor 1/12 (the compact forms fold it
+// into the one scalar that multiplies the derivatives).
+template <typename T, bool kTwelfth, int N, int A0, int A1, int A2, int A3, int A4, int A5>
 LG_DEV void voluder_node(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&dx)[8],
                          T (&dy)[8], T (&dz)[8])
 {
-   const T twelfth = T(1.0) / T(12.0);
-   dx[N] = voluder_form<T, A0, A1, A2, A3, A4, A5>(y, z) * twelfth;
-   dy[N] = -(voluder_form<T, A0, A1, A2, A3, A4, A5>(x, z) * twelfth);
-   dz[N] = -(voluder_form<T, A0, A1, A2, A3, A4, A5>(y, x) * twelfth);
+   if constexpr (kTwelfth) {
+      const T twelfth = T(1.0) / T(12.0);
+      dx[N] = voluder_form<T, A0, A1, A2, A3, A4, A5>(y, z) * twelfth;
+      dy[N] = -(voluder_form<T, A0, A1, A2, A3, A4, A5>(x, z) * twelfth);
+      dz[N] = -(voluder_form<T, A0, A1, A2, A3, A4, A5>(y, x) * twelfth);
+   } else {
+      dx[N] = voluder_form<T, A0, A1, A2, A3, A4, A5>(y, z);
+      dy[N] = -voluder_form<T, A0, A1, A2, A3, A4, A5>(x, z);
+      dz[N] = -voluder_form<T, A0, A1, A2, A3, A4, A5>(y, x);
+   }
 }
 
 // CalcElemVolumeDerivative, lulesh.cc:623-663 (neighbour permutations of
-// each node as the reference passes them to VoluDer).
-template <typename T>
+// each node as the reference passes them to VoluDer); 12x the derivatives
+// when kTwelfth is false.
+template <typename T, bool kTwelfth = true>
 LG_DEV void volume_derivative(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&dx)[8],
                               T (&dy)[8], T (&dz)[8])
 {
-   voluder_node<T, 0, 1, 2, 3, 4, 5, 7>(x, y, z, dx, dy, dz);
-   voluder_node<T, 3, 0, 1, 2, 7, 4, 6>(x, y, z, dx, dy, dz);
-   voluder_node<T, 2, 3, 0, 1, 6, 7, 5>(x, y, z, dx, dy, dz);
-   voluder_node<T, 1, 2, 3, 0, 5, 6, 4>(x, y, z, dx, dy, dz);
-   voluder_node<T, 4, 7, 6, 5, 0, 3, 1>(x, y, z, dx, dy, dz);
-   voluder_node<T, 5, 4, 7, 6, 1, 0, 2>(x, y, z, dx, dy, dz);
-   voluder_node<T, 6, 5, 4, 7, 2, 1, 3>(x, y, z, dx, dy, dz);
-   voluder_node<T, 7, 6, 5, 4, 3, 2, 0>(x, y, z, dx, dy, dz);
+   voluder_node<T, kTwelfth, 0, 1, 2, 3, 4, 5, 7>(x, y, z, dx, dy, dz);
+   voluder_node<T, kTwelfth, 3, 0, 1, 2, 7, 4, 6>(x, y, z, dx, dy, dz);
+   voluder_node<T, kTwelfth, 2, 3, 0, 1, 6, 7, 5>(x, y, z, dx, dy, dz);
+   voluder_node<T, kTwelfth, 1, 2, 3, 0, 5, 6, 4>(x, y, z, dx, dy, dz);
+   voluder_node<T, kTwelfth, 4, 7, 6, 5, 0, 3, 1>(x, y, z, dx, dy, dz);
+   voluder_node<T, kTwelfth, 5, 4, 7, 6, 1, 0, 2>(x, y, z, dx, dy, dz);
+   voluder_node<T, kTwelfth, 6, 5, 4, 7, 2, 1, 3>(x, y, z, dx, dy, dz);
+   voluder_node<T, kTwelfth, 7, 6, 5, 4, 3, 2, 0>(x, y, z, dx, dy, dz);
 }
 
 // Squared-area form of one face, AreaFace (lulesh.cc:1370-1390).
@@ -222,6 +231,114 @@ template <typename T, int A0, int A1, int A2, int A3, int B0, int B1, int B2, in
 LG_DEV T face_diff(const T (&c)[8])
 {
    return (c[A0] + c[A1] + c[A2] + c[A3]) - (c[B0] + c[B1] + c[B2] + c[B3]);
+}
+
+// ---------------------------------------------------------------------------
+// Compact forms for the mixed-precision kinematics.  They are algebraically
+// the reference's formulas, regrouped so that the three element-centred
+// derivative directions share one set of body-diagonal differences; the
+// roundings differ from the reference's, so the bit-exact FP64 forms above
+// do not use them.
+template <typename T>
+struct Vec3 {
+   T x, y, z;
+};
+
+template <typename T>
+LG_DEV Vec3<T> cross(const Vec3<T> &a, const Vec3<T> &b)
+{
+   return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+
+template <typename T>
+LG_DEV T dot(const Vec3<T> &a, const Vec3<T> &b)
+{
+   return a.x * b.x + a.y * b.y + a.z * b.z;
+}
+
+// Derivatives of one coordinate (or velocity component) along xi, eta and
+// zeta at the element centre, unscaled: 8x the Jacobian entries of
+// lulesh.cc:319-329 (fjxxi, fjxet, fjxze), which are also 4x the
+// face-centred differences dxi, dxj, dxk of lulesh.cc:1691-1701.  All three
+// come from the four body diagonals.
+template <typename T>
+LG_DEV Vec3<T> centre_derivs(const T (&c)[8])
+{
+   const T a = c[6] - c[0], b = c[5] - c[3], g = c[7] - c[1], h = c[4] - c[2];
+   const T p = a + b, q = g + h;
+   return {p - q, (a - b) + (g - h), p + q};
+}
+
+// Cross product of the diagonals of face (A, B, C, D): (C - A) x (D - B).
+template <typename T, int A, int B, int C, int D>
+LG_DEV Vec3<T> face_diag_cross(const T (&x)[8], const T (&y)[8], const T (&z)[8])
+{
+   const Vec3<T> d = {x[C] - x[A], y[C] - y[A], z[C] - z[A]};
+   const Vec3<T> e = {x[D] - x[B], y[D] - y[B], z[D] - z[B]};
+   return cross(d, e);
+}
+
+// |(C - A) x (D - B)|^2: AreaFace (lulesh.cc:1370-1390) is
+// |f x g|^2 with f = (C-A) - (D-B), g = (C-A) + (D-B), which is 4x this.
+template <typename T, int A, int B, int C, int D>
+LG_DEV T face_diag_cross2(const T (&x)[8], const T (&y)[8], const T (&z)[8])
+{
+   const Vec3<T> n = face_diag_cross<T, A, B, C, D>(x, y, z);
+   return dot(n, n);
+}
+
+// -8x CalcElemNodeNormals (lulesh.cc:419-474).  SumElemFaceNormal's area
+// vector 0.25 (b0 x b1), with b0 = (p + q)/2 and b1 = (p - q)/2 for the face
+// diagonals p = X2 - X0, q = X3 - X1, is -(p x q)/8; each node sums the
+// three faces it lies on.
+template <typename T>
+LG_DEV void node_normals_diag(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&nx)[8],
+                              T (&ny)[8], T (&nz)[8])
+{
+   const Vec3<T> f0 = face_diag_cross<T, 0, 1, 2, 3>(x, y, z);
+   const Vec3<T> f1 = face_diag_cross<T, 0, 4, 5, 1>(x, y, z);
+   const Vec3<T> f2 = face_diag_cross<T, 1, 5, 6, 2>(x, y, z);
+   const Vec3<T> f3 = face_diag_cross<T, 2, 6, 7, 3>(x, y, z);
+   const Vec3<T> f4 = face_diag_cross<T, 3, 7, 4, 0>(x, y, z);
+   const Vec3<T> f5 = face_diag_cross<T, 4, 7, 6, 5>(x, y, z);
+   auto put = [&](int j, const Vec3<T> &a, const Vec3<T> &b, const Vec3<T> &c) {
+      nx[j] = a.x + b.x + c.x;
+      ny[j] = a.y + b.y + c.y;
+      nz[j] = a.z + b.z + c.z;
+   };
+   put(0, f0, f1, f4);
+   put(1, f0, f1, f2);
+   put(2, f0, f2, f3);
+   put(3, f0, f3, f4);
+   put(4, f1, f4, f5);
+   put(5, f1, f2, f5);
+   put(6, f2, f3, f5);
+   put(7, f3, f4, f5);
+}
+
+// Sign of the centre Jacobian determinant of jacobian_volume: J_eta .
+// (J_zeta x J_xi) of the unscaled centre derivatives is 64x it.
+template <typename T>
+LG_DEV T jacobian_volume_compact(const T (&x)[8], const T (&y)[8], const T (&z)[8])
+{
+   const Vec3<T> cx = centre_derivs<T>(x), cy = centre_derivs<T>(y), cz = centre_derivs<T>(z);
+   const Vec3<T> Ji = {cx.x, cy.x, cz.x}, Jj = {cx.y, cy.y, cz.y}, Jk = {cx.z, cy.z, cz.z};
+   return dot(Jj, cross(Jk, Ji));
+}
+
+// CalcElemCharacteristicLength (lulesh.cc:1394-1435) from the diagonal
+// cross products: 4 V / sqrt(4 m) = 2 V / sqrt(m).
+template <typename T>
+LG_DEV T char_length_diag(const T (&x)[8], const T (&y)[8], const T (&z)[8], T volume)
+{
+   T m = T(0);
+   m = keep_max(face_diag_cross2<T, 0, 1, 2, 3>(x, y, z), m);
+   m = keep_max(face_diag_cross2<T, 4, 5, 6, 7>(x, y, z), m);
+   m = keep_max(face_diag_cross2<T, 0, 1, 5, 4>(x, y, z), m);
+   m = keep_max(face_diag_cross2<T, 1, 2, 6, 5>(x, y, z), m);
+   m = keep_max(face_diag_cross2<T, 2, 3, 7, 6>(x, y, z), m);
+   m = keep_max(face_diag_cross2<T, 3, 0, 4, 7>(x, y, z), m);
+   return div_root(T(2.0) * volume, m);
 }
 
 // Hourglass base vectors, lulesh.cc:743-776: gamma[m][j] = sign bit set

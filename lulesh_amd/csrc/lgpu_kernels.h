@@ -50,6 +50,12 @@ struct PolicyFp64 {
    using nodesum_t = double; // per-node sums of the corner forces
    using pqs_t = double;     // stored p, q, ss, elemMass, arealg, vdov
    static constexpr bool kLocalCoords = false;
+   // geometry in the compact regrouped forms (different roundings) instead
+   // of the reference's association order
+   static constexpr bool kCompact = false;
+   // waves per SIMD (min, max) for the nodal brick kernel's register
+   // allocation and scheduling; (1, 8) leaves it to the compiler
+   static constexpr int kNodalWavesMin = 1, kNodalWavesMax = 8;
    // brick kernel: load the node update's operands before the element work
    static constexpr bool kPrefetchNode = true;
 };
@@ -67,6 +73,10 @@ struct PolicyMixed {
    using nodesum_t = float;
    using pqs_t = float;
    static constexpr bool kLocalCoords = true;
+   static constexpr bool kCompact = true;
+   // exactly 3: aiming for 4 (128 VGPRs) makes the scheduler issue the
+   // hourglass velocity loads one at a time, each waited on at once
+   static constexpr int kNodalWavesMin = 3, kNodalWavesMax = 3;
    static constexpr bool kPrefetchNode = true;
 };
 
