@@ -21,6 +21,8 @@ Prints one JSON line.
              with the origin-energy error against the reference's number
   ensemble   64 Sedov problems (std::mt19937_64(1234) draw), 100 cycles,
              batched vs run one after another
+  setup      problem set-up outside the timed loop: host Domain + device
+             context against the device-side set-up (lgpu_create_sedov)
 """
 from __future__ import annotations
 
@@ -267,6 +269,28 @@ def ensemble_leg(la, torch, members: int, cycles: int, seed: int = 1234) -> dict
     return out
 
 
+def setup_leg(la, torch, s: int, r: int, b: int, c: int) -> dict:
+    """Problem set-up (Domain::Domain, lulesh-init.cc:16-194) outside the timed
+    loop: the host Domain plus its device context (lgpu_create + uploads)
+    against the device-side set-up (lgpu_create_sedov), same problem."""
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    h = la.HostDomain(s, r, b, c)
+    t1 = time.perf_counter()
+    h.attach_device(la.FP64)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    h.close()
+    t3 = time.perf_counter()
+    d = la.SedovDevice(s, r, b, c)
+    torch.cuda.synchronize()
+    t4 = time.perf_counter()
+    d.close()
+    return {"workload": f"sedov s={s} r={r} b={b} c={c}", "host_domain_s": round(t1 - t0, 3),
+            "host_context_s": round(t2 - t1, 3), "device_setup_s": round(t4 - t3, 3),
+            "speedup": round((t2 - t0) / (t4 - t3), 1)}
+
+
 def main() -> None:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -286,6 +310,7 @@ def main() -> None:
     ap.add_argument("--no-mixed", action="store_true")
     ap.add_argument("--no-fma", action="store_true", help="skip the FP64_FMA leg")
     ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--no-setup", action="store_true", help="skip the set-up timing leg")
     ap.add_argument("--ensemble-members", type=int, default=64)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="first run one device copy of 512 MiB (float64), the known-size "
@@ -352,6 +377,9 @@ def main() -> None:
     if not a.no_ensemble:
         print("[bench] ensemble leg", file=sys.stderr, flush=True)
         extra["ensemble"] = ensemble_leg(la, torch, a.ensemble_members, 100)
+    if not a.no_setup:
+        print("[bench] set-up leg", file=sys.stderr, flush=True)
+        extra["setup"] = setup_leg(la, torch, a.size, a.regions, a.balance, a.cost)
 
     cpu = None
     if not a.no_cpu_baseline:

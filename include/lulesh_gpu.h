@@ -111,6 +111,23 @@ void lgpu_default_constants(lgpu_constants *k);
  * except v = 1; upload the rest with lgpu_upload. */
 int lgpu_create(const lgpu_mesh_desc *desc, lgpu_ctx **out);
 
+/* The reference's own problem -- the Sedov blast on an s^3 brick -- described
+ * by its command-line parameters, for set-up on the device. */
+typedef struct lgpu_sedov_desc {
+   int32_t size;                  /* elements per edge (-s)                   */
+   int32_t numReg, balance, cost; /* -r, -b, -c                               */
+   double emult;                  /* initial energy multiplier (1 = -s run)   */
+   /* The region layout as runs of consecutive elements, in element order:
+    * run i gives runLen[i] elements to region runReg[i] (1-based).  These
+    * are the draws of CreateRegionIndexSets (lulesh-init.cc:401-510), a
+    * sequential glibc rand() sequence the host makes (lh_region_runs). */
+   int32_t numRuns;
+   const int32_t *runReg, *runLen;
+   int32_t precision;             /* enum lgpu_precision                      */
+   int32_t flags;                 /* LGPU_FLAG_*                              */
+   lgpu_constants k;
+} lgpu_sedov_desc;
+
 /* Frees the context (Domain::~Domain, lulesh-init.cc:198-213). */
 void lgpu_destroy(lgpu_ctx *ctx);
 
@@ -152,6 +169,17 @@ typedef struct lgpu_time_state {
  * cycle < max_cycle without a host round trip per cycle, then writes the
  * time state back to *ts.  Returns the status as lgpu_time_constraints. */
 int lgpu_run(lgpu_ctx *ctx, lgpu_time_state *ts, int32_t max_cycle);
+
+/* Domain::Domain (lulesh-init.cc:16-194) on the device: builds the context
+ * and, with set-up kernels instead of host loops and uploads, the mesh
+ * (BuildMesh :218-268), the node-to-corner lists
+ * (SetupThreadSupportStructures :272-337), the region work list from the
+ * runs (CreateRegionIndexSets :401-510), symmetry planes, face neighbours and
+ * boundary masks (:514-673), reference volumes and masses (:158-178) and the
+ * energy deposit (:180-192).  The result is bit-identical to lgpu_create +
+ * lgpu_upload of a host-built Domain.  *ts receives the initial time state
+ * (lulesh-init.cc:146-156, deltatime from :192), ready for lgpu_run. */
+int lgpu_create_sedov(const lgpu_sedov_desc *desc, lgpu_ctx **out, lgpu_time_state *ts);
 
 /* Options. */
 enum lgpu_option {

@@ -65,6 +65,14 @@ void lh_symmetry(lh_domain *d, double out[3]);
  * own generator (the source of the region layout, lulesh-init.cc:406-480). */
 void lh_glibc_rand(uint32_t seed, int32_t n, int32_t *out);
 
+/* The region layout of an s^3 problem with nr regions and the given balance
+ * (CreateRegionIndexSets, lulesh-init.cc:401-510) as runs of consecutive
+ * elements, the input of lgpu_create_sedov.  Returns the number of runs;
+ * writes min(that, capacity) of them to reg (1-based region numbers) and len.
+ * Returns -1 on bad arguments. */
+int32_t lh_region_runs(int32_t s, int32_t nr, int32_t balance, int32_t *reg, int32_t *len,
+                       int32_t capacity);
+
 /* ------------------------------------------------------------------------
  * Ensembles of independent Sedov problems (device side: lgpu_ensemble).
  * No counterpart in the reference (one Domain per process there).

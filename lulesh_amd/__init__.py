@@ -5,7 +5,9 @@ the host keeps the reference's Domain class and per-cycle call structure
 (include/lulesh_host.h).  This package only loads those libraries.
 """
 from .native import (FIELDS, FP64, FP64_FMA, MIXED, DeviceContext, Ensemble, HostDomain,  # noqa: F401
-                     TimeState, check, default_constants, ensemble_draw, gpu_lib, host_lib)
+                     SedovDevice, TimeState, check, default_constants, ensemble_draw, gpu_lib,
+                     host_lib, region_runs)
 
-__all__ = ["FIELDS", "FP64", "FP64_FMA", "MIXED", "DeviceContext", "Ensemble", "HostDomain", "TimeState",
-           "check", "default_constants", "ensemble_draw", "gpu_lib", "host_lib"]
+__all__ = ["FIELDS", "FP64", "FP64_FMA", "MIXED", "DeviceContext", "Ensemble", "HostDomain",
+           "SedovDevice", "TimeState", "check", "default_constants", "ensemble_draw", "gpu_lib",
+           "host_lib", "region_runs"]

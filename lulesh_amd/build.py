@@ -40,6 +40,7 @@ HIP_SOURCES = {
     "lgpu_step_mixed.hip": ["-ffp-contract=fast"],
     "lgpu_step_fp64fma.hip": ["-ffp-contract=fast"],
     "lgpu_ensemble.hip": ["-ffp-contract=off"],
+    "lgpu_setup.hip": ["-ffp-contract=off"],
 }
 HOST_SOURCES = ["host/domain.cc", "host/glibc_rand.cc"]
 

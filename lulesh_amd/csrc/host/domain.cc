@@ -23,6 +23,61 @@ enum : Int_t {
 
 } // namespace
 
+namespace lhost {
+
+// The region draws of CreateRegionIndexSets (lulesh-init.cc:401-480) as runs
+// of consecutive elements: region numbers (1-based) drawn with weights
+// (i+1)^balance and never twice in a row, run lengths from the reference's
+// bins, the last run cut at numElem.  One region: a single run.
+void draw_region_runs(Index_t numElem, Int_t nr, Int_t balance, std::vector<Int_t> &runReg,
+                      std::vector<Int_t> &runLen)
+{
+   runReg.clear();
+   runLen.clear();
+   if (nr == 1) {
+      runReg.push_back(1);
+      runLen.push_back(numElem);
+      return;
+   }
+   lhost::GlibcRand rng(0);
+   // Cumulative weights (i+1)^balance.
+   std::vector<Int_t> binEnd((size_t)nr);
+   Int_t total = 0;
+   for (Int_t i = 0; i < nr; ++i) {
+      total += pow(i + 1, balance);
+      binEnd[(size_t)i] = total;
+   }
+   auto draw_region = [&]() {
+      const Int_t v = rng.next() % total;
+      Int_t i = 0;
+      while (v >= binEnd[(size_t)i]) ++i;
+      return (i % nr) + 1;
+   };
+   Index_t next = 0;
+   Int_t last = -1;
+   while (next < numElem) {
+      Int_t reg = draw_region();
+      while (reg == last) reg = draw_region();
+      // Run length by bin (lulesh-init.cc:460-480).
+      const Int_t bin = rng.next() % 1000;
+      Index_t run;
+      if (bin < 773) run = rng.next() % 15 + 1;
+      else if (bin < 937) run = rng.next() % 16 + 16;
+      else if (bin < 970) run = rng.next() % 32 + 32;
+      else if (bin < 974) run = rng.next() % 64 + 64;
+      else if (bin < 978) run = rng.next() % 128 + 128;
+      else if (bin < 981) run = rng.next() % 256 + 256;
+      else run = rng.next() % 1537 + 512;
+      const Index_t end = std::min(next + run, numElem);
+      runReg.push_back(reg);
+      runLen.push_back(end - next);
+      next = end;
+      last = reg;
+   }
+}
+
+} // namespace lhost
+
 Domain::Domain(Int_t numRanks, Index_t colLoc, Index_t rowLoc, Index_t planeLoc, Index_t nx,
                Int_t tp, Int_t nr, Int_t balance, Int_t cost)
 {
@@ -172,46 +227,14 @@ void Domain::SetupThreadSupportStructures()
 // rotation), drawn from the glibc sequence.
 void Domain::CreateRegionIndexSets(Int_t nr, Int_t balance)
 {
-   lhost::GlibcRand rng(0);
    m_numReg = nr;
    m_regElemSize.assign((size_t)nr, 0);
    m_regElemlist.assign((size_t)nr, {});
+   std::vector<Int_t> runReg, runLen;
+   lhost::draw_region_runs(m_numElem, nr, balance, runReg, runLen);
    Index_t next = 0;
-   if (nr == 1) {
-      std::fill(m_regNumList.begin(), m_regNumList.end(), 1);
-   } else {
-      // Cumulative weights (i+1)^balance.
-      std::vector<Int_t> binEnd((size_t)nr);
-      Int_t total = 0;
-      for (Int_t i = 0; i < nr; ++i) {
-         total += pow(i + 1, balance);
-         binEnd[(size_t)i] = total;
-      }
-      auto draw_region = [&]() {
-         const Int_t v = rng.next() % total;
-         Int_t i = 0;
-         while (v >= binEnd[(size_t)i]) ++i;
-         return (i % nr) + 1;
-      };
-      Int_t last = -1;
-      while (next < m_numElem) {
-         Int_t reg = draw_region();
-         while (reg == last) reg = draw_region();
-         // Run length by bin (lulesh-init.cc:460-480).
-         const Int_t bin = rng.next() % 1000;
-         Index_t run;
-         if (bin < 773) run = rng.next() % 15 + 1;
-         else if (bin < 937) run = rng.next() % 16 + 16;
-         else if (bin < 970) run = rng.next() % 32 + 32;
-         else if (bin < 974) run = rng.next() % 64 + 64;
-         else if (bin < 978) run = rng.next() % 128 + 128;
-         else if (bin < 981) run = rng.next() % 256 + 256;
-         else run = rng.next() % 1537 + 512;
-         const Index_t end = std::min(next + run, m_numElem);
-         for (; next < end; ++next) m_regNumList[(size_t)next] = reg;
-         last = reg;
-      }
-   }
+   for (size_t r = 0; r < runReg.size(); ++r)
+      for (Index_t i = 0; i < runLen[r]; ++i) m_regNumList[(size_t)next++] = runReg[r];
    for (Index_t i = 0; i < m_numElem; ++i) m_regElemSize[(size_t)(m_regNumList[(size_t)i] - 1)]++;
    for (Int_t r = 0; r < nr; ++r) m_regElemlist[(size_t)r].reserve((size_t)m_regElemSize[(size_t)r]);
    for (Index_t i = 0; i < m_numElem; ++i) m_regElemlist[(size_t)(m_regNumList[(size_t)i] - 1)].push_back(i);
@@ -654,6 +677,22 @@ void lh_glibc_rand(uint32_t seed, int32_t n, int32_t *out)
 {
    lhost::GlibcRand g(seed);
    for (int32_t i = 0; i < n; ++i) out[i] = g.next();
+}
+
+int32_t lh_region_runs(int32_t s, int32_t nr, int32_t balance, int32_t *reg, int32_t *len,
+                       int32_t capacity)
+{
+   if (s < 1 || nr < 1 || balance < 0 || (int64_t)s * s * s > INT32_MAX || capacity < 0 ||
+       (capacity > 0 && (!reg || !len)))
+      return -1;
+   std::vector<Int_t> runReg, runLen;
+   lhost::draw_region_runs(s * s * s, nr, balance, runReg, runLen);
+   const int32_t n = (int32_t)runReg.size();
+   for (int32_t i = 0; i < std::min(n, capacity); ++i) {
+      reg[i] = runReg[(size_t)i];
+      len[i] = runLen[(size_t)i];
+   }
+   return n;
 }
 
 void lh_ensemble_draw(uint64_t seed, int32_t n, int32_t *s, double *emult, int32_t *balance,

@@ -18,6 +18,7 @@
 #include "lgpu_brick.h"
 #include "lgpu_kernels.h"
 #include "lgpu_launch.h"
+#include "lgpu_setup.h"
 #include "lulesh_gpu.h"
 
 using namespace lgpu;
@@ -207,8 +208,6 @@ int32_t region_rep(int32_t r, int32_t numReg, int32_t cost)
    return 10 * (1 + cost);
 }
 
-// Device array of a field.  `narrow`: stored as FP32 in the mixed form (the
-// Q gradients, p, q, ss, elemMass, arealg and vdov).
 // k_eos dispatch order.  k = 0: work-list order (largest repeat count
 // first).  k > 0: the workgroups of the largest repeat count are spread out,
 // one every k + 1 dispatches, with the others (in list order) between them,
@@ -241,6 +240,8 @@ int set_eos_order(lgpu_ctx *c, int k)
    return LGPU_OK;
 }
 
+// Device array of a field.  `narrow`: stored as FP32 in the mixed form (the
+// Q gradients, p, q, ss, elemMass, arealg and vdov).
 double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &narrow)
 {
    isElem = f >= LGPU_E;
@@ -389,6 +390,176 @@ __global__ void k_time_final(DevTime *t, const double *dtOut)
    }
 }
 
+// What the context's arena and kernel path depend on.
+struct CtxSpec {
+   int32_t numElem = 0, numNode = 0, sizeX = 0, numReg = 0, cost = 0, precision = 0;
+   bool brick = false, brickFaces = false;
+   int64_t nCorner = 0; // length of the node-to-corner list
+   lgpu_constants k{};
+};
+
+// Builds the context: stream, one zeroed arena with every array of the step,
+// the brick tiling.  Tables and state are filled by the caller.
+int alloc_context(const CtxSpec &sp, lgpu_ctx **out)
+{
+   lgpu_ctx *c = new (std::nothrow) lgpu_ctx();
+   if (!c) return LGPU_ENOMEM;
+   auto fail = [&](int code) {
+      lgpu_destroy(c);
+      return code;
+   };
+   int dev = 0;
+   if (hipGetDevice(&dev) != hipSuccess) return fail(LGPU_EDEVICE);
+   c->device = dev;
+   c->numElem = sp.numElem;
+   c->numNode = sp.numNode;
+   c->numReg = sp.numReg;
+   c->cost = sp.cost;
+   c->precision = sp.precision;
+   c->sizeX = sp.sizeX;
+   const lgpu_constants &kk = sp.k;
+   c->k = Consts{kk.e_cut, kk.p_cut, kk.q_cut, kk.v_cut, kk.u_cut, kk.hgcoef, kk.ss4o3,
+                 kk.qstop, kk.monoq_max_slope, kk.monoq_limiter_mult, kk.qlc_monoq,
+                 kk.qqc_monoq, kk.qqc, kk.eosvmax, kk.eosvmin, kk.pmin, kk.emin, kk.dvovmax,
+                 kk.refdens};
+   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
+      return fail(LGPU_EDEVICE);
+
+   const bool brick = sp.brick;
+   const size_t E = (size_t)sp.numElem, N = (size_t)sp.numNode;
+   const size_t gradBytes = (sp.precision == LGPU_MIXED) ? 4 : 8;
+   const size_t cornerBytes = (sp.precision == LGPU_MIXED) ? 4 : 8;
+   const int64_t plane = (int64_t)(align_up(E * cornerBytes) / cornerBytes);
+   c->planeStride = plane;
+   const int64_t nCorner = sp.nCorner;
+   const unsigned eosBlocks = eos_blocks(sp.numElem);
+
+   // Arena layout.
+   struct Slot { void **dst; size_t bytes; };
+   std::vector<Slot> slots;
+   DevState &s = c->s;
+   auto add = [&](void **dst, size_t bytes) { slots.push_back({dst, align_up(bytes)}); };
+   double **nodeReal[] = {&s.x, &s.y, &s.z, &s.xd, &s.yd, &s.zd, &s.nodalMass};
+   for (double **p : nodeReal) add((void **)p, sizeof(double) * N);
+   if (brick) {
+      double **second[] = {&s.xo, &s.yo, &s.zo, &s.xdo, &s.ydo, &s.zdo};
+      for (double **p : second) add((void **)p, sizeof(double) * N);
+   }
+   add((void **)&s.nodeBC, N);
+   add((void **)&s.nodeElemStart, sizeof(int32_t) * (N + 1));
+   add((void **)&s.cornerList, sizeof(int32_t) * (size_t)std::max<int64_t>(nCorner, 1));
+   add((void **)&s.nodelist, sizeof(int32_t) * 8 * E);
+   const int32_t **nbr[] = {&s.lxim, &s.lxip, &s.letam, &s.letap, &s.lzetam, &s.lzetap, &s.elemBC};
+   for (const int32_t **p : nbr) add((void **)p, sizeof(int32_t) * E);
+   void **pqsArrays[] = {&s.p, &s.q, &s.ss, &s.elemMass, &s.arealg, &s.vdov};
+   for (void **p : pqsArrays) add(p, gradBytes * E);
+   double **elemReal[] = {&s.e, &s.v, &s.volo, &s.vnew, &s.delv};
+   for (double **p : elemReal) add((void **)p, sizeof(double) * E);
+   void **grads[] = {&s.delx_xi, &s.delx_eta, &s.delx_zeta, &s.delv_xi, &s.delv_eta, &s.delv_zeta};
+   for (void **p : grads) add(p, gradBytes * E);
+   if (!brick) add(&s.corner, cornerBytes * (size_t)plane * 48);
+   add((void **)&s.perm, sizeof(int32_t) * E);
+   add((void **)&s.eosBlockOrder, sizeof(int32_t) * eosBlocks);
+   add((void **)&s.regStart, sizeof(int32_t) * (size_t)(sp.numReg + 1));
+   add((void **)&s.regRep, sizeof(int32_t) * (size_t)sp.numReg);
+   add((void **)&s.err, sizeof(int32_t) * kNumErrSites);
+   add((void **)&s.dtPartial, sizeof(double) * 2 * eosBlocks);
+   add((void **)&s.dtOut, sizeof(double) * 2);
+   add((void **)&c->devTime, sizeof(DevTime));
+   size_t total = 0;
+   for (const Slot &sl : slots) total += sl.bytes;
+   if (hipMalloc((void **)&c->arena, total) != hipSuccess) {
+      (void)hipGetLastError();
+      fprintf(stderr, "lgpu_create: device arena of %zu bytes failed\n", total);
+      return fail(LGPU_ENOMEM);
+   }
+   c->arenaBytes = total;
+   if (hipMemset(c->arena, 0, total) != hipSuccess) return fail(LGPU_EDEVICE);
+   size_t off = 0;
+   for (const Slot &sl : slots) {
+      *sl.dst = c->arena + off;
+      off += sl.bytes;
+   }
+   s.numElem = sp.numElem;
+   s.numNode = sp.numNode;
+   s.numReg = sp.numReg;
+   s.numPerm = sp.numElem;
+   s.xcdRemap = 1;
+   s.isBrick = brick ? 1 : 0;
+   s.brickFaces = (brick && sp.brickFaces) ? 1 : 0;
+   if (brick) {
+      Brick &g = s.brick;
+      init_brick(g, sp.sizeX);
+      int cus = 0;
+      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+         cus = 0;
+      int perCu = 0;
+      switch (sp.precision) {
+      case LGPU_MIXED: perCu = Launch<PolicyMixed>::nodal_brick_blocks_per_cu(); break;
+      case LGPU_FP64_FMA: perCu = Launch<PolicyFp64Fma>::nodal_brick_blocks_per_cu(); break;
+      default: perCu = Launch<PolicyFp64>::nodal_brick_blocks_per_cu(); break;
+      }
+      c->brickPerCu = perCu;
+      set_brick_chunk(g, pick_brick_chunk(g, cus * perCu));
+   }
+   *out = c;
+   return LGPU_OK;
+}
+
+// The EOS work-list groups: one per distinct repeat count, largest first,
+// with their sizes; the elements of group i are perm[regStart[i] ..
+// regStart[i+1]).  Uploads regStart/regRep and derives the repeat count of
+// each k_eos block (the largest over the groups it holds).
+int set_groups(lgpu_ctx *c, const std::vector<int32_t> &reps, const std::vector<int64_t> &sizes)
+{
+   DevState &s = c->s;
+   const int numGroups = std::max<int>(1, (int)reps.size());
+   std::vector<int32_t> regStart((size_t)numGroups + 1, 0), regRep((size_t)numGroups, 1);
+   int64_t at = 0;
+   for (size_t k = 0; k < reps.size(); ++k) {
+      regStart[k] = (int32_t)at;
+      regRep[k] = reps[k];
+      at += sizes[k];
+   }
+   regStart[(size_t)numGroups] = (int32_t)at;
+   if (at != c->numElem) return LGPU_EINVAL;
+   s.numReg = numGroups;
+   const unsigned eosBlocks = eos_blocks(c->numElem);
+   c->eosBlockRep.assign(eosBlocks, 1);
+   for (int i = 0; i < numGroups; ++i)
+      for (int64_t p = regStart[(size_t)i]; p < regStart[(size_t)i + 1]; p += kEosBlock) {
+         const size_t b = (size_t)(p / kEosBlock);
+         c->eosBlockRep[b] = std::max(c->eosBlockRep[b], regRep[(size_t)i]);
+      }
+   for (int i = 0; i < numGroups; ++i)
+      if (regStart[(size_t)i + 1] > regStart[(size_t)i]) {
+         const size_t b = (size_t)((regStart[(size_t)i + 1] - 1) / kEosBlock);
+         c->eosBlockRep[b] = std::max(c->eosBlockRep[b], regRep[(size_t)i]);
+      }
+   HIP_OR_RETURN(c, hipMemcpy(const_cast<int32_t *>(s.regStart), regStart.data(),
+                              sizeof(int32_t) * regStart.size(), hipMemcpyHostToDevice));
+   HIP_OR_RETURN(c, hipMemcpy(const_cast<int32_t *>(s.regRep), regRep.data(),
+                              sizeof(int32_t) * regRep.size(), hipMemcpyHostToDevice));
+   return LGPU_OK;
+}
+
+// Last steps of both constructors: the EOS dispatch order and the pinned
+// read-back buffers.
+int finish_context(lgpu_ctx *c)
+{
+   // One workgroup of the largest repeat count every 5 dispatches: measured
+   // 7-10% faster than plain work-list order at s=192 (FP64, 11 and 21
+   // regions).
+   const int st = set_eos_order(c, 4);
+   if (st != LGPU_OK) return st;
+   if (hipHostMalloc((void **)&c->hostDt, sizeof(double) * 2 + sizeof(int32_t) * kNumErrSites +
+                                               sizeof(DevTime) + 64, hipHostMallocDefault) != hipSuccess)
+      return LGPU_ENOMEM;
+   c->hostErr = reinterpret_cast<int32_t *>(c->hostDt + 2);
+   c->hostTime = reinterpret_cast<DevTime *>(reinterpret_cast<char *>(c->hostDt) + 64);
+   return LGPU_OK;
+}
+
 } // namespace
 
 extern "C" {
@@ -426,106 +597,27 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
       fprintf(stderr, "lgpu_create: invalid mesh description: %s\n", why.c_str());
       return LGPU_EINVAL;
    }
-   lgpu_ctx *c = new (std::nothrow) lgpu_ctx();
-   if (!c) return LGPU_ENOMEM;
+   CtxSpec sp;
+   sp.numElem = d->numElem;
+   sp.numNode = d->numNode;
+   sp.sizeX = d->sizeX;
+   sp.numReg = d->numReg;
+   sp.cost = d->cost;
+   sp.precision = d->precision;
+   sp.brick = !(d->flags & LGPU_FLAG_GENERIC) && is_reference_brick(d);
+   sp.brickFaces = sp.brick && brick_faces_computable(d);
+   sp.nCorner = d->nodeElemStart[d->numNode];
+   sp.k = d->k;
+   lgpu_ctx *c = nullptr;
+   int st = alloc_context(sp, &c);
+   if (st != LGPU_OK) return st;
    auto fail = [&](int code) {
       lgpu_destroy(c);
       return code;
    };
-   int dev = 0;
-   if (hipGetDevice(&dev) != hipSuccess) return fail(LGPU_EDEVICE);
-   c->device = dev;
-   c->numElem = d->numElem;
-   c->numNode = d->numNode;
-   c->numReg = d->numReg;
-   c->cost = d->cost;
-   c->precision = d->precision;
-   c->sizeX = d->sizeX;
-   const lgpu_constants &kk = d->k;
-   c->k = Consts{kk.e_cut, kk.p_cut, kk.q_cut, kk.v_cut, kk.u_cut, kk.hgcoef, kk.ss4o3,
-                 kk.qstop, kk.monoq_max_slope, kk.monoq_limiter_mult, kk.qlc_monoq,
-                 kk.qqc_monoq, kk.qqc, kk.eosvmax, kk.eosvmin, kk.pmin, kk.emin, kk.dvovmax,
-                 kk.refdens};
-   if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
-      return fail(LGPU_EDEVICE);
-
-   const bool brick = !(d->flags & LGPU_FLAG_GENERIC) && is_reference_brick(d);
-   const size_t E = (size_t)d->numElem, N = (size_t)d->numNode;
-   const size_t gradBytes = (d->precision == LGPU_MIXED) ? 4 : 8;
-   const size_t cornerBytes = (d->precision == LGPU_MIXED) ? 4 : 8;
-   const int64_t plane = (int64_t)(align_up(E * cornerBytes) / cornerBytes);
-   c->planeStride = plane;
-   const int64_t nCorner = d->nodeElemStart[N];
-   const unsigned eosBlocks = eos_blocks(d->numElem);
-
-   // Arena layout.
-   struct Slot { void **dst; size_t bytes; };
-   std::vector<Slot> slots;
    DevState &s = c->s;
-   auto add = [&](void **dst, size_t bytes) { slots.push_back({dst, align_up(bytes)}); };
-   double **nodeReal[] = {&s.x, &s.y, &s.z, &s.xd, &s.yd, &s.zd, &s.nodalMass};
-   for (double **p : nodeReal) add((void **)p, sizeof(double) * N);
-   if (brick) {
-      double **second[] = {&s.xo, &s.yo, &s.zo, &s.xdo, &s.ydo, &s.zdo};
-      for (double **p : second) add((void **)p, sizeof(double) * N);
-   }
-   add((void **)&s.nodeBC, N);
-   add((void **)&s.nodeElemStart, sizeof(int32_t) * (N + 1));
-   add((void **)&s.cornerList, sizeof(int32_t) * (size_t)std::max<int64_t>(nCorner, 1));
-   add((void **)&s.nodelist, sizeof(int32_t) * 8 * E);
-   const int32_t **nbr[] = {&s.lxim, &s.lxip, &s.letam, &s.letap, &s.lzetam, &s.lzetap, &s.elemBC};
-   for (const int32_t **p : nbr) add((void **)p, sizeof(int32_t) * E);
-   void **pqsArrays[] = {&s.p, &s.q, &s.ss, &s.elemMass, &s.arealg, &s.vdov};
-   for (void **p : pqsArrays) add(p, gradBytes * E);
-   double **elemReal[] = {&s.e, &s.v, &s.volo, &s.vnew, &s.delv};
-   for (double **p : elemReal) add((void **)p, sizeof(double) * E);
-   void **grads[] = {&s.delx_xi, &s.delx_eta, &s.delx_zeta, &s.delv_xi, &s.delv_eta, &s.delv_zeta};
-   for (void **p : grads) add(p, gradBytes * E);
-   if (!brick) add(&s.corner, cornerBytes * (size_t)plane * 48);
-   add((void **)&s.perm, sizeof(int32_t) * E);
-   add((void **)&s.eosBlockOrder, sizeof(int32_t) * eosBlocks);
-   add((void **)&s.regStart, sizeof(int32_t) * (size_t)(d->numReg + 1));
-   add((void **)&s.regRep, sizeof(int32_t) * (size_t)d->numReg);
-   add((void **)&s.err, sizeof(int32_t) * kNumErrSites);
-   add((void **)&s.dtPartial, sizeof(double) * 2 * eosBlocks);
-   add((void **)&s.dtOut, sizeof(double) * 2);
-   add((void **)&c->devTime, sizeof(DevTime));
-   size_t total = 0;
-   for (const Slot &sl : slots) total += sl.bytes;
-   if (hipMalloc((void **)&c->arena, total) != hipSuccess) {
-      (void)hipGetLastError();
-      fprintf(stderr, "lgpu_create: device arena of %zu bytes failed\n", total);
-      return fail(LGPU_ENOMEM);
-   }
-   c->arenaBytes = total;
-   if (hipMemset(c->arena, 0, total) != hipSuccess) return fail(LGPU_EDEVICE);
-   size_t off = 0;
-   for (const Slot &sl : slots) {
-      *sl.dst = c->arena + off;
-      off += sl.bytes;
-   }
-   s.numElem = d->numElem;
-   s.numNode = d->numNode;
-   s.numReg = d->numReg;
-   s.numPerm = d->numElem;
-   s.xcdRemap = 1;
-   s.isBrick = brick ? 1 : 0;
-   s.brickFaces = (brick && brick_faces_computable(d)) ? 1 : 0;
-   if (brick) {
-      Brick &g = s.brick;
-      init_brick(g, d->sizeX);
-      int cus = 0;
-      if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
-         cus = 0;
-      int perCu = 0;
-      switch (d->precision) {
-      case LGPU_MIXED: perCu = Launch<PolicyMixed>::nodal_brick_blocks_per_cu(); break;
-      case LGPU_FP64_FMA: perCu = Launch<PolicyFp64Fma>::nodal_brick_blocks_per_cu(); break;
-      default: perCu = Launch<PolicyFp64>::nodal_brick_blocks_per_cu(); break;
-      }
-      c->brickPerCu = perCu;
-      set_brick_chunk(g, pick_brick_chunk(g, cus * perCu));
-   }
+   const size_t E = (size_t)d->numElem, N = (size_t)d->numNode;
+   const int64_t nCorner = sp.nCorner;
 
    // Host-built tables.
    std::vector<uint8_t> bc(N, 0);
@@ -552,28 +644,13 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
       for (int32_t i = 0; i < d->regElemSize[r]; ++i) repOf[(size_t)d->regElemlist[r][i]] = rep;
    }
    std::sort(reps.begin(), reps.end(), std::greater<int32_t>());
-   const int numGroups = std::max<int>(1, (int)reps.size());
-   std::vector<int32_t> regStart((size_t)numGroups + 1, 0), regRep((size_t)numGroups, 1);
-   for (size_t k = 0; k < reps.size(); ++k) {
-      regStart[k] = (int32_t)perm.size();
-      regRep[k] = reps[k];
+   std::vector<int64_t> sizes;
+   for (int32_t rep : reps) {
+      const size_t before = perm.size();
       for (size_t el = 0; el < E; ++el)
-         if (repOf[el] == reps[k]) perm.push_back((int32_t)el);
+         if (repOf[el] == rep) perm.push_back((int32_t)el);
+      sizes.push_back((int64_t)(perm.size() - before));
    }
-   regStart[(size_t)numGroups] = (int32_t)perm.size();
-   s.numReg = numGroups;
-   // Repeat count of each k_eos block: the largest over the groups it holds.
-   c->eosBlockRep.assign(eosBlocks, 1);
-   for (int i = 0; i < numGroups; ++i)
-      for (int64_t p = regStart[(size_t)i]; p < regStart[(size_t)i + 1]; p += kEosBlock) {
-         const size_t b = (size_t)(p / kEosBlock);
-         c->eosBlockRep[b] = std::max(c->eosBlockRep[b], regRep[(size_t)i]);
-      }
-   for (int i = 0; i < numGroups; ++i)
-      if (regStart[(size_t)i + 1] > regStart[(size_t)i]) {
-         const size_t b = (size_t)((regStart[(size_t)i + 1] - 1) / kEosBlock);
-         c->eosBlockRep[b] = std::max(c->eosBlockRep[b], regRep[(size_t)i]);
-      }
    std::vector<double> ones(E, 1.0);
 
    auto h2d = [&](const void *dst, const void *src, size_t bytes) {
@@ -588,19 +665,129 @@ int lgpu_create(const lgpu_mesh_desc *d, lgpu_ctx **out)
              h2d(s.lzetam, d->lzetam, sizeof(int32_t) * E) && h2d(s.lzetap, d->lzetap, sizeof(int32_t) * E) &&
              h2d(s.elemBC, d->elemBC, sizeof(int32_t) * E) &&
              h2d(s.perm, perm.data(), sizeof(int32_t) * E) &&
-             h2d(s.regStart, regStart.data(), sizeof(int32_t) * regStart.size()) &&
-             h2d(s.regRep, regRep.data(), sizeof(int32_t) * regRep.size()) &&
              h2d(s.v, ones.data(), sizeof(double) * E);
    if (!ok) return fail(LGPU_EDEVICE);
-   // One workgroup of the largest repeat count every 5 dispatches: measured
-   // 7-10% faster than plain region order at s=192 (FP64, 11 and 21 regions).
-   if (set_eos_order(c, 4) != LGPU_OK) return fail(LGPU_EDEVICE);
+   if ((st = set_groups(c, reps, sizes)) != LGPU_OK) return fail(st);
+   if ((st = finish_context(c)) != LGPU_OK) return fail(st);
+   *out = c;
+   return LGPU_OK;
+}
 
-   if (hipHostMalloc((void **)&c->hostDt, sizeof(double) * 2 + sizeof(int32_t) * kNumErrSites +
-                                               sizeof(DevTime) + 64, hipHostMallocDefault) != hipSuccess)
+int lgpu_create_sedov(const lgpu_sedov_desc *d, lgpu_ctx **out, lgpu_time_state *ts)
+{
+   if (!out) return LGPU_EINVAL;
+   *out = nullptr;
+   auto invalid = [](const char *why) {
+      fprintf(stderr, "lgpu_create_sedov: invalid description: %s\n", why);
+      return LGPU_EINVAL;
+   };
+   if (!d || !ts) return invalid("null argument");
+   const int64_t sz = d->size, E = sz * sz * sz, N = (sz + 1) * (sz + 1) * (sz + 1);
+   if (sz < 1 || 8 * E > INT32_MAX || N > INT32_MAX) return invalid("size out of range");
+   if (d->precision != LGPU_FP64 && d->precision != LGPU_MIXED && d->precision != LGPU_FP64_FMA)
+      return invalid("precision");
+   if (d->numReg < 1 || d->cost < 0 || d->balance < 0) return invalid("numReg/balance/cost");
+   if (!(d->emult > 0.0)) return invalid("emult");
+   if (d->numRuns < 1 || !d->runReg || !d->runLen) return invalid("region runs");
+   // Runs: region of each, start of each, and the groups of equal repeat count.
+   std::vector<int32_t> runStart((size_t)d->numRuns + 1, 0);
+   std::vector<int64_t> regSize((size_t)d->numReg, 0);
+   int64_t at = 0;
+   for (int32_t r = 0; r < d->numRuns; ++r) {
+      if (d->runReg[r] < 1 || d->runReg[r] > d->numReg || d->runLen[r] < 1)
+         return invalid("region run");
+      runStart[(size_t)r] = (int32_t)at;
+      at += d->runLen[r];
+      if (at > E) return invalid("region runs exceed the element count");
+      regSize[(size_t)(d->runReg[r] - 1)] += d->runLen[r];
+   }
+   if (at != E) return invalid("region runs do not cover every element");
+   runStart[(size_t)d->numRuns] = (int32_t)E;
+   std::vector<int32_t> reps;
+   for (int r = 0; r < d->numReg; ++r) {
+      const int32_t rep = region_rep(r, d->numReg, d->cost);
+      if (regSize[(size_t)r] > 0 && std::find(reps.begin(), reps.end(), rep) == reps.end())
+         reps.push_back(rep);
+   }
+   std::sort(reps.begin(), reps.end(), std::greater<int32_t>());
+   std::vector<int64_t> groupSize(reps.size(), 0), groupAt(reps.size(), 0);
+   auto group_of = [&](int32_t reg) {
+      const int32_t rep = region_rep(reg - 1, d->numReg, d->cost);
+      return (size_t)(std::find(reps.begin(), reps.end(), rep) - reps.begin());
+   };
+   for (int r = 0; r < d->numReg; ++r)
+      if (regSize[(size_t)r] > 0) groupSize[group_of(r + 1)] += regSize[(size_t)r];
+   for (size_t g = 1; g < reps.size(); ++g) groupAt[g] = groupAt[g - 1] + groupSize[g - 1];
+   std::vector<int32_t> runDest((size_t)d->numRuns);
+   for (int32_t r = 0; r < d->numRuns; ++r) {
+      const size_t g = group_of(d->runReg[r]);
+      runDest[(size_t)r] = (int32_t)groupAt[g];
+      groupAt[g] += d->runLen[r];
+   }
+
+   CtxSpec sp;
+   sp.numElem = (int32_t)E;
+   sp.numNode = (int32_t)N;
+   sp.sizeX = (int32_t)sz;
+   sp.numReg = d->numReg;
+   sp.cost = d->cost;
+   sp.precision = d->precision;
+   sp.brick = !(d->flags & LGPU_FLAG_GENERIC);
+   sp.brickFaces = sp.brick;
+   sp.nCorner = 8 * E;
+   sp.k = d->k;
+   lgpu_ctx *c = nullptr;
+   int st = alloc_context(sp, &c);
+   if (st != LGPU_OK) return st;
+   int32_t *runTables = nullptr;
+   auto fail = [&](int code) {
+      if (runTables) (void)hipFree(runTables);
+      lgpu_destroy(c);
+      return code;
+   };
+   const size_t runBytes = sizeof(int32_t) * ((size_t)d->numRuns + 1);
+   if (hipMalloc((void **)&runTables, 2 * runBytes) != hipSuccess) {
+      (void)hipGetLastError();
       return fail(LGPU_ENOMEM);
-   c->hostErr = reinterpret_cast<int32_t *>(c->hostDt + 2);
-   c->hostTime = reinterpret_cast<DevTime *>(reinterpret_cast<char *>(c->hostDt) + 64);
+   }
+   int32_t *dStart = runTables, *dDest = runTables + d->numRuns + 1;
+   if (hipMemcpy(dStart, runStart.data(), runBytes, hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(dDest, runDest.data(), sizeof(int32_t) * (size_t)d->numRuns,
+                 hipMemcpyHostToDevice) != hipSuccess)
+      return fail(LGPU_EDEVICE);
+   SedovSetup t;
+   t.size = (int32_t)sz;
+   t.runStart = dStart;
+   t.runDest = dDest;
+   t.numRuns = d->numRuns;
+   t.narrowMass = d->precision == LGPU_MIXED;
+   if (launch_sedov_setup(c->s, t, c->stream) != hipSuccess) return fail(LGPU_EDEVICE);
+   if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(LGPU_EDEVICE);
+   (void)hipFree(runTables);
+   runTables = nullptr;
+   if ((st = set_groups(c, reps, groupSize)) != LGPU_OK) return fail(st);
+   if ((st = finish_context(c)) != LGPU_OK) return fail(st);
+
+   // Energy deposit and first dt (lulesh-init.cc:180-192).
+   const double ebase = 3.948746e+7;
+   const double scale = (double)sz / 45.0;
+   double einit = ebase * scale * scale * scale;
+   if (d->emult != 1.0) einit = einit * d->emult;
+   double volo0 = 0.0;
+   if (hipMemcpy(c->s.e, &einit, sizeof(double), hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(&volo0, c->s.volo, sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+      return fail(LGPU_EDEVICE);
+   // Time controls, lulesh-init.cc:146-156.
+   ts->time = 0.0;
+   ts->deltatime = (0.5 * cbrt(volo0)) / sqrt(2.0 * einit);
+   ts->dtcourant = 1.0e+20;
+   ts->dthydro = 1.0e+20;
+   ts->stoptime = 1.0e-2;
+   ts->dtfixed = -1.0e-6;
+   ts->dtmax = 1.0e-2;
+   ts->deltatimemultlb = 1.1;
+   ts->deltatimemultub = 1.2;
+   ts->cycle = 0;
    *out = c;
    return LGPU_OK;
 }

@@ -58,6 +58,15 @@ class MeshDesc(ctypes.Structure):
     ]
 
 
+class SedovDesc(ctypes.Structure):
+    _fields_ = [
+        ("size", ctypes.c_int32), ("numReg", ctypes.c_int32), ("balance", ctypes.c_int32),
+        ("cost", ctypes.c_int32), ("emult", ctypes.c_double), ("numRuns", ctypes.c_int32),
+        ("runReg", _i32p), ("runLen", _i32p), ("precision", ctypes.c_int32),
+        ("flags", ctypes.c_int32), ("k", Constants),
+    ]
+
+
 class TimeState(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in (
         "time", "deltatime", "dtcourant", "dthydro", "stoptime", "dtfixed", "dtmax",
@@ -89,6 +98,9 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lgpu_default_constants.restype = None
         lib.lgpu_create.argtypes = [ctypes.POINTER(MeshDesc), ctypes.POINTER(vp)]
         lib.lgpu_create.restype = ctypes.c_int
+        lib.lgpu_create_sedov.argtypes = [ctypes.POINTER(SedovDesc), ctypes.POINTER(vp),
+                                          ctypes.POINTER(TimeState)]
+        lib.lgpu_create_sedov.restype = ctypes.c_int
         lib.lgpu_destroy.argtypes = [vp]
         lib.lgpu_destroy.restype = None
         for n in ("lgpu_upload", "lgpu_download"):
@@ -183,6 +195,8 @@ def host_lib() -> ctypes.CDLL:
         lib.lh_symmetry.restype = None
         lib.lh_glibc_rand.argtypes = [ctypes.c_uint32, i32, _i32p]
         lib.lh_glibc_rand.restype = None
+        lib.lh_region_runs.argtypes = [i32, i32, i32, _i32p, _i32p, i32]
+        lib.lh_region_runs.restype = i32
         lib.lh_ensemble_draw.argtypes = [ctypes.c_uint64, i32, _i32p, _f64p, _i32p, _i32p]
         lib.lh_ensemble_draw.restype = None
         lib.lh_ensemble_create.argtypes = [i32, _i32p, _f64p, _i32p, _i32p, _i32p, i32]
@@ -323,6 +337,53 @@ class DeviceContext:
         i = Info()
         check(gpu_lib().lgpu_get_info(self.ptr, ctypes.byref(i)), self.ptr)
         return i
+
+
+def region_runs(s: int, nr: int, balance: int) -> tuple[np.ndarray, np.ndarray]:
+    """The region layout as runs (lh_region_runs): 1-based region numbers and
+    run lengths in element order."""
+    lib = host_lib()
+    n = lib.lh_region_runs(s, nr, balance, None, None, 0)
+    if n < 1:
+        raise ValueError("lh_region_runs: bad arguments")
+    reg, ln = np.zeros(n, np.int32), np.zeros(n, np.int32)
+    lib.lh_region_runs(s, nr, balance, reg.ctypes.data_as(_i32p), ln.ctypes.data_as(_i32p), n)
+    return reg, ln
+
+
+class SedovDevice(DeviceContext):
+    """The reference's Sedov problem set up on the device (lgpu_create_sedov)
+    instead of from a host Domain; stepped with the device-side loop."""
+
+    def __init__(self, s: int, nr: int = 11, balance: int = 1, cost: int = 1,
+                 emult: float = 1.0, precision: int = FP64, flags: int = 0,
+                 constants: Constants | None = None):
+        lib = gpu_lib()
+        reg, ln = region_runs(s, nr, balance)
+        d = SedovDesc()
+        d.size, d.numReg, d.balance, d.cost, d.emult = s, nr, balance, cost, float(emult)
+        d.numRuns = len(reg)
+        d.runReg = reg.ctypes.data_as(_i32p)
+        d.runLen = ln.ctypes.data_as(_i32p)
+        d.precision, d.flags = precision, flags
+        d.k = constants if constants is not None else default_constants()
+        self.ts = TimeState()
+        ptr = ctypes.c_void_p()
+        check(lib.lgpu_create_sedov(ctypes.byref(d), ctypes.byref(ptr), ctypes.byref(self.ts)))
+        self.ptr = ptr
+        self.s = s
+        self.numElem, self.numNode = s ** 3, (s + 1) ** 3
+        self.precision = precision
+
+    def run(self, its: int) -> int:
+        """Advances up to `its` more cycles (or to stoptime) on the device."""
+        return super().run(self.ts, self.ts.cycle + its)
+
+    def get(self, name: str) -> float:
+        return getattr(self.ts, name)
+
+    def real(self, name: str) -> np.ndarray:
+        return self.download(name)
 
 
 INT_ARRAYS = ["nodelist", "lxim", "lxip", "letam", "letap", "lzetam", "lzetap", "elemBC",
