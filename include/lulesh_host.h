@@ -61,6 +61,12 @@ void lh_set(lh_domain *d, int32_t which, double value);
  * (lulesh-util.cc:197-212) on the host copy of e. */
 void lh_symmetry(lh_domain *d, double out[3]);
 
+/* The end-of-run visualization dump (DumpToVisit, lulesh-viz.cc:279-364) as a
+ * legacy binary VTK file: mesh, regions, e, p, v, q per element and speed,
+ * xd, yd, zd per node, from the host copies (call lh_sync_from_device
+ * first).  0 on success, -1 if the file cannot be written. */
+int lh_write_vtk(lh_domain *d, const char *path);
+
 /* The first n values of glibc's rand() after srand(seed), from the host's
  * own generator (the source of the region layout, lulesh-init.cc:406-480). */
 void lh_glibc_rand(uint32_t seed, int32_t n, int32_t *out);

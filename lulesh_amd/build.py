@@ -42,7 +42,7 @@ HIP_SOURCES = {
     "lgpu_ensemble.hip": ["-ffp-contract=off"],
     "lgpu_setup.hip": ["-ffp-contract=off"],
 }
-HOST_SOURCES = ["host/domain.cc", "host/glibc_rand.cc"]
+HOST_SOURCES = ["host/domain.cc", "host/glibc_rand.cc", "host/viz.cc"]
 
 
 def _run(cmd: list[str]) -> None:

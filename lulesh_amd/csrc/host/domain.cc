@@ -654,6 +654,12 @@ void lh_set(lh_domain *h, int32_t which, double v)
    }
 }
 
+int lh_write_vtk(lh_domain *h, const char *path)
+{
+   if (!h || !path) return -1;
+   return WriteVtk(*h->d, path);
+}
+
 void lh_symmetry(lh_domain *h, double out[3])
 {
    // lulesh-util.cc:197-212 over the host copy of e.

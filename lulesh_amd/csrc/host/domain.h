@@ -166,6 +166,9 @@ class Domain {
 void TimeIncrement(Domain &domain);
 void LagrangeNodal(Domain &domain);
 void LagrangeElements(Domain &domain, Index_t numElem);
+// End-of-run visualization dump (DumpToVisit, lulesh-viz.cc) as legacy VTK
+// (host/viz.cc); writes the host copies.  0 on success.
+int WriteVtk(Domain &domain, const char *path);
 void CalcTimeConstraintsForElems(Domain &domain);
 void LagrangeLeapFrog(Domain &domain);
 

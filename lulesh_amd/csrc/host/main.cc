@@ -2,8 +2,8 @@
 // over the device path, with its flags and its end-of-run report
 // (lulesh-util.cc:175-230).
 //
-// Flags as the reference (-q -i -s -r -b -c -p -h; -f/-v accepted and
-// ignored) plus:
+// Flags as the reference (-q -i -s -r -b -c -p -v -h; -f accepted and
+// ignored; -v writes the end-of-run dump as legacy VTK, host/viz.cc) plus:
 //   -P fp64|fp64fma|mixed   device precision (default fp64)
 //   -D              run TimeIncrement on the device (no per-cycle host sync)
 #include <stdio.h>
@@ -22,7 +22,7 @@ namespace {
 
 struct Opts {
    int its = 9999999, nx = 30, numReg = 11, balance = 1, cost = 1;
-   int showProg = 0, quiet = 0, precision = LGPU_FP64, deviceLoop = 0;
+   int showProg = 0, quiet = 0, precision = LGPU_FP64, deviceLoop = 0, viz = 0;
 };
 
 void usage(const char *exe)
@@ -35,6 +35,7 @@ void usage(const char *exe)
    printf(" -b <balance>    : Load balance between regions of a domain (def: 1)\n");
    printf(" -c <cost>       : Extra cost of more expensive regions (def: 1)\n");
    printf(" -p              : Print out progress\n");
+   printf(" -v              : Output viz file (lulesh_plot_c<cycle>.vtk, legacy VTK)\n");
    printf(" -P fp64|fp64fma|mixed : device precision (def: fp64)\n");
    printf(" -D              : time step control on the device\n");
    printf(" -h              : This message\n");
@@ -69,7 +70,7 @@ Opts parse(int argc, char **argv)
       else if (!strcmp(argv[i], "-f")) (void)int_arg(argc, argv, i);
       else if (!strcmp(argv[i], "-p")) { o.showProg = 1; ++i; }
       else if (!strcmp(argv[i], "-q")) { o.quiet = 1; ++i; }
-      else if (!strcmp(argv[i], "-v")) { ++i; }
+      else if (!strcmp(argv[i], "-v")) { o.viz = 1; ++i; }
       else if (!strcmp(argv[i], "-D")) { o.deviceLoop = 1; ++i; }
       else if (!strcmp(argv[i], "-P")) {
          if (i + 1 >= argc) { printf("Missing argument to -P\n"); exit(-1); }
@@ -142,6 +143,12 @@ int main(int argc, char **argv)
    if (d.SyncFromDevice()) {
       fprintf(stderr, "device read-back failed\n");
       return 1;
+   }
+   if (o.viz) {
+      // Final viz file, lulesh.cc:2776-2779.
+      char name[64];
+      snprintf(name, sizeof(name), "lulesh_plot_c%d.vtk", d.cycle());
+      if (WriteVtk(d, name)) fprintf(stderr, "could not write %s\n", name);
    }
 
    if (!o.quiet) {

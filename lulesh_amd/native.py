@@ -193,6 +193,8 @@ def host_lib() -> ctypes.CDLL:
         lib.lh_set.restype = None
         lib.lh_symmetry.argtypes = [vp, _f64p]
         lib.lh_symmetry.restype = None
+        lib.lh_write_vtk.argtypes = [vp, ctypes.c_char_p]
+        lib.lh_write_vtk.restype = ctypes.c_int
         lib.lh_glibc_rand.argtypes = [ctypes.c_uint32, i32, _i32p]
         lib.lh_glibc_rand.restype = None
         lib.lh_region_runs.argtypes = [i32, i32, i32, _i32p, _i32p, i32]
@@ -454,6 +456,11 @@ class HostDomain:
         out = (ctypes.c_double * 3)()
         self._lib.lh_symmetry(self.ptr, out)
         return out[0], out[1], out[2]
+
+    def write_vtk(self, path) -> None:
+        """The end-of-run visualization dump (legacy VTK) of the host copies."""
+        if self._lib.lh_write_vtk(self.ptr, str(path).encode()) != 0:
+            raise OSError(f"could not write {path}")
 
 
 def ensemble_draw(n: int, seed: int = 1234):
