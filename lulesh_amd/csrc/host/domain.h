@@ -166,6 +166,13 @@ class Domain {
 void TimeIncrement(Domain &domain);
 void LagrangeNodal(Domain &domain);
 void LagrangeElements(Domain &domain, Index_t numElem);
+namespace lhost {
+// The region draws of CreateRegionIndexSets (lulesh-init.cc:401-480) as runs
+// of consecutive elements: region numbers (1-based) and run lengths.
+void draw_region_runs(Index_t numElem, Int_t nr, Int_t balance, std::vector<Int_t> &runReg,
+                      std::vector<Int_t> &runLen);
+} // namespace lhost
+
 // End-of-run visualization dump (DumpToVisit, lulesh-viz.cc) as legacy VTK
 // (host/viz.cc); writes the host copies.  0 on success.
 int WriteVtk(Domain &domain, const char *path);
