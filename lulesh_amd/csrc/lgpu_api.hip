@@ -19,6 +19,7 @@
 #include "lgpu_kernels.h"
 #include "lgpu_launch.h"
 #include "lgpu_setup.h"
+#include "lgpu_time.h"
 #include "lulesh_gpu.h"
 
 using namespace lgpu;
@@ -40,13 +41,6 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 struct TimedLaunch {
    int kernel;
    hipEvent_t start, stop;
-};
-
-// Device-side time state for lgpu_run: TimeIncrement operands plus the loop
-// flag the kernels test.
-struct DevTime {
-   double time, deltatime, dtcourant, dthydro, stoptime, dtfixed, dtmax, multlb, multub;
-   int32_t cycle, maxCycle, active, pad;
 };
 
 } // namespace
@@ -343,41 +337,12 @@ int resolve_status(const int32_t *err)
    return LGPU_OK;
 }
 
-// TimeIncrement (lulesh.cc:166-222) for the device-side loop; also decides
-// whether this cycle runs at all (the loop test of lulesh.cc:2745).
+// TimeIncrement for the first cycle of a device-side loop; later cycles'
+// increments are folded into k_dt_reduce (lgpu_time.h).
 __global__ void k_time_increment(DevTime *t, const double *dtOut, const int32_t *err)
 {
    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-   if (!t->active) return;
-   bool failed = false;
-   for (int i = 0; i < kNumErrSites; ++i) failed = failed || (err[i] != 0);
-   if (t->cycle > 0) {
-      t->dtcourant = dtOut[0];
-      t->dthydro = dtOut[1];
-   }
-   if (failed || !(t->time < t->stoptime && t->cycle < t->maxCycle)) {
-      t->active = 0;
-      return;
-   }
-   double targetdt = t->stoptime - t->time;
-   if (t->dtfixed <= 0.0 && t->cycle != 0) {
-      const double olddt = t->deltatime;
-      double newdt = 1.0e+20;
-      if (t->dtcourant < newdt) newdt = t->dtcourant / 2.0;
-      if (t->dthydro < newdt) newdt = t->dthydro * 2.0 / 3.0;
-      const double ratio = newdt / olddt;
-      if (ratio >= 1.0) {
-         if (ratio < t->multlb) newdt = olddt;
-         else if (ratio > t->multub) newdt = olddt * t->multub;
-      }
-      if (newdt > t->dtmax) newdt = t->dtmax;
-      t->deltatime = newdt;
-   }
-   if (targetdt > t->deltatime && targetdt < 4.0 * t->deltatime / 3.0)
-      targetdt = 2.0 * t->deltatime / 3.0;
-   if (targetdt < t->deltatime) t->deltatime = targetdt;
-   t->time += t->deltatime;
-   t->cycle += 1;
+   time_increment(t, dtOut[0], dtOut[1], err);
 }
 
 // Folds the last cycle's dt results into the time state once the loop ends.
@@ -916,20 +881,25 @@ int lgpu_run(lgpu_ctx *c, lgpu_time_state *ts, int32_t maxCycle)
 
    c->s.dtDevice = &c->devTime->deltatime;
    c->s.active = &c->devTime->active;
+   c->s.time = c->devTime; // k_dt_reduce runs the next cycle's TimeIncrement
    // Cycles are enqueued in batches between host checks of the loop flag.
    // No more cycles are enqueued than the cycle limit allows, so only a run
    // that ends at stoptime (or on an error) enqueues no-op cycles.
    int64_t remaining = (int64_t)maxCycle - (int64_t)ts->cycle;
    int st = LGPU_OK;
+   bool first = true;
    for (;;) {
       const int batch = (int)std::max<int64_t>(1, std::min<int64_t>(16, remaining));
       remaining -= batch;
       for (int b = 0; b < batch; ++b) {
-         st = timed(c, kKTimeIncrement, [&] {
-            hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, c->stream, c->devTime,
-                               c->s.dtOut, c->s.err);
-         });
-         if (st) break;
+         if (first) {
+            st = timed(c, kKTimeIncrement, [&] {
+               hipLaunchKernelGGL(k_time_increment, dim3(1), dim3(64), 0, c->stream, c->devTime,
+                                  c->s.dtOut, c->s.err);
+            });
+            if (st) break;
+            first = false;
+         }
          st = lgpu_lagrange_nodal(c, 0.0);
          if (st) break;
          st = lgpu_lagrange_elements(c, 0.0);
@@ -944,6 +914,7 @@ int lgpu_run(lgpu_ctx *c, lgpu_time_state *ts, int32_t maxCycle)
    // Back to host-driven dt (the node buffers keep whatever swaps happened).
    c->s.dtDevice = nullptr;
    c->s.active = nullptr;
+   c->s.time = nullptr;
    if (st) return st;
    hipLaunchKernelGGL(k_time_final, dim3(1), dim3(64), 0, c->stream, c->devTime, c->s.dtOut);
    HIP_OR_RETURN(c, hipGetLastError());

@@ -141,6 +141,9 @@ struct DevState {
    double *dtOut;           // [0] dtcourant, [1] dthydro
    const double *dtDevice;  // device-side deltatime (null: use argument)
    const int32_t *active;   // device-side loop flag (null: always active)
+   // device-side time state: k_dt_reduce also runs the next cycle's
+   // TimeIncrement (null: host-driven time stepping)
+   struct DevTime *time;
    Brick brick;             // valid when isBrick
    int32_t isBrick;
    int32_t brickFaces; // k_eos computes face neighbours and BC masks from the brick
