@@ -95,13 +95,14 @@ inline void set_brick_chunk(Brick &g, int chunk)
 // chunk + 1 element planes for chunk node planes, and the launch runs in
 // rounds of `slots` resident workgroups; pick the chunk that minimises
 // rounds x (chunk + 1), i.e. redundant planes against a partly filled last
-// round (s=192 on 256 CUs: 13).
+// round.  s=192 on 256 CUs: 65 for FP64 (2 workgroups per CU: all 507
+// resident at once, 2% faster than 22), 22 for the mixed form (3 per CU).
 inline int pick_brick_chunk(const Brick &g, int slots)
 {
    if (slots < 1) return 8;
    int best = 8;
    double bestCost = 1e300;
-   for (int ch = 4; ch <= 32 && ch <= g.en; ++ch) {
+   for (int ch = 4; ch <= g.en; ++ch) {
       const int64_t blocks = (int64_t)g.tilesX * g.tilesY * ((g.en + ch - 1) / ch);
       const int64_t rounds = (blocks + slots - 1) / slots;
       const double cost = (double)rounds * (ch + 1);
