@@ -22,6 +22,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "lgpu_geom.h"
 
 namespace lgpu {
@@ -58,6 +60,9 @@ struct PolicyFp64 {
    static constexpr int kNodalWavesMin = 1, kNodalWavesMax = 8;
    // brick kernel: load the node update's operands before the element work
    static constexpr bool kPrefetchNode = true;
+   // brick kernel: 32-bit byte offsets from scalar bases (at32) instead of
+   // 64-bit per-lane addresses; FP64 runs 1.4% slower with them (more VGPRs)
+   static constexpr bool kOffset32 = false;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
@@ -78,7 +83,20 @@ struct PolicyMixed {
    // hourglass velocity loads one at a time, each waited on at once
    static constexpr int kNodalWavesMin = 3, kNodalWavesMax = 3;
    static constexpr bool kPrefetchNode = true;
+   static constexpr bool kOffset32 = true;
 };
+
+// Element i of a device array through a 32-bit byte offset: the access
+// compiles to the scalar-base + 32-bit vector-offset form of the global
+// load/store instead of per-lane 64-bit address arithmetic.  The caller
+// guarantees i * sizeof(T) < 2^32 (the brick kernels: lgpu_create admits at
+// most 2^28 elements, so a brick has fewer than 2^28.1 nodes).
+template <typename T>
+__device__ __forceinline__ T &at32(T *base, uint32_t i)
+{
+   using B = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+   return *reinterpret_cast<T *>(reinterpret_cast<B *>(base) + i * (uint32_t)sizeof(T));
+}
 
 // Division by a run-time constant d (< 2^32) as a 64-bit multiply-high:
 // q = hi64(n * m) with m = floor((2^64 - 1) / d) + 1 is exact for n < 2^32.
