@@ -63,6 +63,9 @@ struct PolicyFp64 {
    // brick kernel: 32-bit byte offsets from scalar bases (at32) instead of
    // 64-bit per-lane addresses; FP64 runs 1.4% slower with them (more VGPRs)
    static constexpr bool kOffset32 = false;
+   // kinematics brick kernel: fetch the next node plane one plane ahead
+   // (24 more VGPRs; FP64 would drop from 3 to 2 waves/SIMD and run 8% slower)
+   static constexpr bool kKinPrefetch = false;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
@@ -84,6 +87,7 @@ struct PolicyMixed {
    static constexpr int kNodalWavesMin = 3, kNodalWavesMax = 3;
    static constexpr bool kPrefetchNode = true;
    static constexpr bool kOffset32 = true;
+   static constexpr bool kKinPrefetch = true; // 118 VGPRs, 4 waves/SIMD: 3% faster
 };
 
 // Element i of a device array through a 32-bit byte offset: the access
