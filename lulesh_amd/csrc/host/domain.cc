@@ -302,7 +302,11 @@ namespace {
 // constants volo, elemMass, nodalMass).
 const int kStateFields[] = {LGPU_X, LGPU_Y, LGPU_Z, LGPU_XD, LGPU_YD, LGPU_ZD, LGPU_NODALMASS,
                             LGPU_E, LGPU_P, LGPU_Q, LGPU_V, LGPU_VOLO, LGPU_SS, LGPU_ELEMMASS};
-const int kDerivedFields[] = {LGPU_VNEW, LGPU_DELV, LGPU_VDOV, LGPU_AREALG};
+// What a cycle writes: the fields a sync brings back.  nodalMass, volo and
+// elemMass stay the host's FP64 inputs (the mixed form keeps elemMass in FP32
+// on the device; its download would be the rounded copy).
+const int kStepFields[] = {LGPU_X, LGPU_Y, LGPU_Z, LGPU_XD, LGPU_YD, LGPU_ZD, LGPU_E, LGPU_P,
+                           LGPU_Q, LGPU_V, LGPU_SS, LGPU_VNEW, LGPU_DELV, LGPU_VDOV, LGPU_AREALG};
 } // namespace
 
 int Domain::AttachDevice(int precision, int flags)
@@ -354,11 +358,7 @@ int Domain::AttachDevice(int precision, int flags)
 int Domain::SyncFromDevice()
 {
    if (!m_gpu) return LGPU_EINVAL;
-   for (int f : kStateFields) {
-      const int st = lgpu_download(m_gpu, f, realField(f));
-      if (st) return st;
-   }
-   for (int f : kDerivedFields) {
+   for (int f : kStepFields) {
       const int st = lgpu_download(m_gpu, f, realField(f));
       if (st) return st;
    }
@@ -775,10 +775,8 @@ int lh_ensemble_run(lh_ensemble *e, int32_t its, int32_t *status)
 int lh_ensemble_sync(lh_ensemble *e)
 {
    if (!e) return LGPU_EINVAL;
-   static const int kDown[] = {LGPU_X, LGPU_Y, LGPU_Z, LGPU_XD, LGPU_YD, LGPU_ZD, LGPU_E, LGPU_P,
-                               LGPU_Q, LGPU_V, LGPU_SS, LGPU_VNEW, LGPU_DELV, LGPU_VDOV, LGPU_AREALG};
    for (size_t i = 0; i < e->members.size(); ++i)
-      for (int f : kDown) {
+      for (int f : kStepFields) {
          const int st = lgpu_ensemble_download(e->dev, (int32_t)i, f, e->members[i]->realField(f));
          if (st) return st;
       }

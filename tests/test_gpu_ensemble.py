@@ -49,6 +49,27 @@ def test_members_equal_single_runs(graph, generic):
             assert np.array_equal(mem.real(f), ref.real(f)), (i, f)
 
 
+def test_mixed_members_equal_single_mixed_runs():
+    """The mixed form on the brick path: each workgroup shifts the state
+    pointers to its member and addresses from there with 32-bit offsets, the
+    kinematics kernel prefetches node planes across member boundaries of the
+    grid; every member still equals its own single-problem mixed run bit for
+    bit (same kernels, same operation order, per-member dt)."""
+    ens = la.Ensemble(**SMALL, precision=la.MIXED)
+    assert ens.brick
+    st, status = ens.run(25)
+    assert st == 0 and not status.any()
+    ens.sync()
+    for i, mem in enumerate(ens.members):
+        s, m = SMALL["sizes"][i], SMALL["emults"][i]
+        b, c = SMALL["balances"][i], SMALL["costs"][i]
+        ref = _single(s, b, c, m, 25, precision=la.MIXED)
+        assert mem.get("cycle") == ref.get("cycle")
+        assert mem.get("deltatime") == ref.get("deltatime")
+        for f in STATE_FIELDS:
+            assert np.array_equal(mem.real(f), ref.real(f)), (i, f)
+
+
 def test_members_stop_at_their_own_end():
     """s=2,3,4 reach t=1e-2 after 5, 23, 45 cycles (reference); an s=6 member
     runs on to the cycle limit."""
