@@ -39,7 +39,9 @@ int lh_step(lh_domain *d);
  * the error code. */
 int lh_run(lh_domain *d, int32_t its, int32_t device_loop);
 
-/* Copies every state field from the device into the host Domain. */
+/* Copies the fields a cycle writes (x, y, z, xd, yd, zd, e, p, q, v, ss,
+ * vnew, delv, vdov, arealg) from the device into the host Domain; nodalMass,
+ * volo and elemMass keep the host's FP64 values. */
 int lh_sync_from_device(lh_domain *d);
 
 int32_t lh_num_elem(lh_domain *d);
@@ -106,7 +108,8 @@ void lh_ensemble_destroy(lh_ensemble *e);
  * (n entries).  Returns 0 or the first member error. */
 int lh_ensemble_run(lh_ensemble *e, int32_t its, int32_t *status);
 
-/* Copies every member's state back into its host Domain. */
+/* Copies the fields a cycle writes (as lh_sync_from_device) of every member
+ * back into its host Domain. */
 int lh_ensemble_sync(lh_ensemble *e);
 
 /* Host Domain of member m, owned by the ensemble (do not lh_destroy it). */
