@@ -189,6 +189,27 @@ def test_fixed_cycle_runs_match_reference(key):
     assert abs(e0 - float(ref["e0"])) <= 1e-10 * float(ref["e0"]), (e0, ref["e0"])
 
 
+@pytest.mark.parametrize("key", [(96, 11, 1, 1, 100, 1.0), (192, 11, 1, 1, 200, 1.0)],
+                         ids=["s96", "s192"])
+def test_full_size_runs_match_reference(key):
+    """The reference's s=96 and s=192 runs (host-built Domain, device loop):
+    iteration count, origin energy within 1e-10 and the plane-0 symmetry
+    diffs of the same order as the reference's own."""
+    s, r, b, c, its, m = key
+    ref = summary(*key)
+    h = _run_host(s, r, b, c, its, device_loop=True)
+    assert int(h.get("cycle")) == ref["cycles"] == its
+    e0 = h.real("e")[0]
+    err = abs(e0 - float(ref["e0"])) / float(ref["e0"])
+    sym = h.symmetry()
+    print(f"s={s}: e0 {e0:.17e} (ref {ref['e0']}, rel err {err:.1e}) sym {sym}")
+    assert err <= 1e-10
+    for got, name in zip(sym, ("maxabs", "totabs", "maxrel")):
+        want = float(ref[name])
+        assert want / 10 <= got <= want * 10, (name, got, want)
+    h.close()
+
+
 def test_region_layout_does_not_change_results():
     a = _run_host(16, 1, 1, 1, 50)
     b = _run_host(16, 21, 2, 2, 50)
