@@ -466,6 +466,13 @@ int alloc_context(const CtxSpec &sp, lgpu_ctx **out)
       }
       c->brickPerCu = perCu;
       set_brick_chunk(g, pick_brick_chunk(g, cus * perCu));
+      int kinPerCu = 0;
+      switch (sp.precision) {
+      case LGPU_MIXED: kinPerCu = Launch<PolicyMixed>::kin_brick_blocks_per_cu(); break;
+      case LGPU_FP64_FMA: kinPerCu = Launch<PolicyFp64Fma>::kin_brick_blocks_per_cu(); break;
+      default: kinPerCu = Launch<PolicyFp64>::kin_brick_blocks_per_cu(); break;
+      }
+      set_kin_chunk(g, pick_kin_chunk(g, cus * kinPerCu));
    }
    *out = c;
    return LGPU_OK;

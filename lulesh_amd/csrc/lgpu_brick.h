@@ -120,6 +120,27 @@ inline void set_kin_chunk(Brick &g, int chunk)
    g.kinChunks = (g.s + g.kinChunk - 1) / g.kinChunk;
 }
 
+// Default element planes per workgroup of the kinematics kernel, by the same
+// rule: a workgroup loads chunk + 1 node planes for chunk element planes.
+// s=192 on 256 CUs: 12 for FP64 (3 workgroups per CU), 14 for the mixed form
+// (4 per CU: 2% faster than 12).
+inline int pick_kin_chunk(const Brick &g, int slots)
+{
+   if (slots < 1) return 12;
+   int best = 12;
+   double bestCost = 1e300;
+   for (int ch = 4; ch <= g.s; ++ch) {
+      const int64_t blocks = (int64_t)g.kinTiles * g.kinTiles * ((g.s + ch - 1) / ch);
+      const int64_t rounds = (blocks + slots - 1) / slots;
+      const double cost = (double)rounds * (ch + 1);
+      if (cost < bestCost) {
+         bestCost = cost;
+         best = ch;
+      }
+   }
+   return best;
+}
+
 // Tiling of an s^3 brick with the default chunks (the caller may pick
 // others): 15x15-node tiles of the fused nodal kernel, 16x16-element tiles
 // of the kinematics kernel.

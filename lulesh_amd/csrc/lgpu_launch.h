@@ -28,6 +28,8 @@ struct Launch {
    static int nodal_brick_blocks_per_cu();
    static void kin(const DevState &s, const Consts &k, double dt, int block, hipStream_t st);
    static void kin_brick(const DevState &s, const Consts &k, double dt, hipStream_t st);
+   // resident k_kin_brick workgroups per CU (0 if the query fails)
+   static int kin_brick_blocks_per_cu();
    // ensemble on the brick path: one launch over every member's workgroups
    // (16x16-thread tiles), the grid sizes from the member tables
    static void nodal_brick_members(const DevState &s, const Consts &k, unsigned grid,
