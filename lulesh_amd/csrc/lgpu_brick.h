@@ -91,19 +91,18 @@ inline void set_brick_chunk(Brick &g, int chunk)
    g.chunks = (g.en + g.chunkZ - 1) / g.chunkZ;
 }
 
-// Default node planes per workgroup of the brick kernel.  A workgroup walks
-// chunk + 1 element planes for chunk node planes, and the launch runs in
-// rounds of `slots` resident workgroups; pick the chunk that minimises
-// rounds x (chunk + 1), i.e. redundant planes against a partly filled last
-// round.  s=192 on 256 CUs: 65 for FP64 (2 workgroups per CU: all 507
-// resident at once, 2% faster than 22), 22 for the mixed form (3 per CU).
-inline int pick_brick_chunk(const Brick &g, int slots)
+// Planes per workgroup of a column-walking kernel: a launch of `columns`
+// column tiles, each cut into z-chunks of `chunk` planes out of `planes`,
+// runs in rounds of `slots` resident workgroups, and a workgroup walks one
+// plane more than its chunk.  Picks the chunk that minimises rounds x
+// (chunk + 1): redundant planes against a partly filled last round.
+inline int pick_chunk(int64_t columns, int planes, int slots, int fallback)
 {
-   if (slots < 1) return 8;
-   int best = 8;
+   if (slots < 1) return fallback;
+   int best = fallback;
    double bestCost = 1e300;
-   for (int ch = 4; ch <= g.en; ++ch) {
-      const int64_t blocks = (int64_t)g.tilesX * g.tilesY * ((g.en + ch - 1) / ch);
+   for (int ch = 4; ch <= planes; ++ch) {
+      const int64_t blocks = columns * ((planes + ch - 1) / ch);
       const int64_t rounds = (blocks + slots - 1) / slots;
       const double cost = (double)rounds * (ch + 1);
       if (cost < bestCost) {
@@ -112,6 +111,14 @@ inline int pick_brick_chunk(const Brick &g, int slots)
       }
    }
    return best;
+}
+
+// Default node planes per workgroup of the fused nodal kernel.  s=192 on
+// 256 CUs: 65 for FP64 (2 workgroups per CU: all 507 resident at once, 2%
+// faster than 22), 22 for the mixed form (3 per CU).
+inline int pick_brick_chunk(const Brick &g, int slots)
+{
+   return pick_chunk((int64_t)g.tilesX * g.tilesY, g.en, slots, 8);
 }
 
 inline void set_kin_chunk(Brick &g, int chunk)
@@ -120,25 +127,12 @@ inline void set_kin_chunk(Brick &g, int chunk)
    g.kinChunks = (g.s + g.kinChunk - 1) / g.kinChunk;
 }
 
-// Default element planes per workgroup of the kinematics kernel, by the same
-// rule: a workgroup loads chunk + 1 node planes for chunk element planes.
-// s=192 on 256 CUs: 12 for FP64 (3 workgroups per CU), 14 for the mixed form
-// (4 per CU: 2% faster than 12).
+// Default element planes per workgroup of the kinematics kernel.  s=192 on
+// 256 CUs: 12 for FP64 (3 workgroups per CU), 14 for the mixed form (4 per
+// CU: 3% faster than 12).
 inline int pick_kin_chunk(const Brick &g, int slots)
 {
-   if (slots < 1) return 12;
-   int best = 12;
-   double bestCost = 1e300;
-   for (int ch = 4; ch <= g.s; ++ch) {
-      const int64_t blocks = (int64_t)g.kinTiles * g.kinTiles * ((g.s + ch - 1) / ch);
-      const int64_t rounds = (blocks + slots - 1) / slots;
-      const double cost = (double)rounds * (ch + 1);
-      if (cost < bestCost) {
-         bestCost = cost;
-         best = ch;
-      }
-   }
-   return best;
+   return pick_chunk((int64_t)g.kinTiles * g.kinTiles, g.s, slots, 12);
 }
 
 // Tiling of an s^3 brick with the default chunks (the caller may pick
