@@ -17,7 +17,7 @@ import pytest
 
 import lulesh_amd as la
 from helpers import STATE_FIELDS, context_from_oracle, max_rel, summary, upload_state
-from oracle_binding import OracleDomain
+from oracle_binding import OracleDomain, have_reference_build, read_dump, run_refdump
 
 pytestmark = pytest.mark.gpu
 
@@ -207,6 +207,29 @@ def test_full_size_runs_match_reference(key):
     for got, name in zip(sym, ("maxabs", "totabs", "maxrel")):
         want = float(ref[name])
         assert want / 10 <= got <= want * 10, (name, got, want)
+    h.close()
+
+
+@pytest.mark.skipif(not have_reference_build(), reason="oracle/_ref not built")
+def test_s96_fields_match_reference_build(tmp_path):
+    """s=96, 11 regions, 100 cycles: every state field of the device run
+    (host-built Domain, device loop) against the reference's own OpenMP build
+    at the same cycle (its dump harness, oracle/_ref/refdump).  Fields agree
+    to 1e-10 of their scale: only the hourglass cube root may differ, by an
+    ulp, from glibc's."""
+    dump = tmp_path / "s96.bin"
+    ref = run_refdump(["-s", 96, "-r", 11, "-b", 1, "-c", 1, "-i", 100], dump=str(dump),
+                      threads=16)
+    want = read_dump(dump)
+    h = _run_host(96, 11, 1, 1, 100, device_loop=True)
+    assert int(h.get("cycle")) == ref["cycles"] == 100
+    assert abs(h.get("time") - float(ref["time"])) <= 1e-12 * float(ref["time"])
+    worst = {}
+    for f in STATE_FIELDS:
+        worst[f] = max_rel(h.real(f), want["final/" + f])
+    print("s=96 per-field max |gpu - ref| / scale:", {k: f"{v:.1e}" for k, v in worst.items()})
+    for f, err in worst.items():
+        assert err <= 1e-10, (f, err)
     h.close()
 
 
