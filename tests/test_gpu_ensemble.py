@@ -4,7 +4,9 @@ its own dt, advanced by one captured cycle graph.
 FP64 members must equal single-problem runs of the same member bit for bit
 (both follow the reference's operation order), end where their own loop test
 ends them, and reproduce the reference's end-of-run numbers.  Mixed members
-stay within the mixed form's bound of the FP64 reference.
+equal single-problem mixed runs bit for bit and stay within the mixed form's
+bound of the FP64 reference.  The benchmark's own 64-member draw is checked
+member by member in both precisions.
 """
 import numpy as np
 import pytest
@@ -68,6 +70,28 @@ def test_mixed_members_equal_single_mixed_runs():
         assert mem.get("deltatime") == ref.get("deltatime")
         for f in STATE_FIELDS:
             assert np.array_equal(mem.real(f), ref.real(f)), (i, f)
+
+
+@pytest.mark.parametrize("precision", [la.FP64, la.MIXED], ids=["fp64", "mixed"])
+def test_benchmark_ensemble_members_equal_single_runs(precision):
+    """The benchmark's ensemble itself: 64 members drawn from
+    std::mt19937_64(1234) (s in 16..64, energy multiplier in [0.5, 2],
+    balance and cost in {1, 2}), 100 cycles with the autotuner's choice;
+    every member equals its own single-problem run bit for bit."""
+    sz, em, b, c = la.ensemble_draw(64, 1234)
+    ens = la.Ensemble(sizes=sz, emults=em, balances=b, costs=c, precision=precision)
+    ens.autotune(trial_cycles=2)
+    st, status = ens.run(100)
+    assert st == 0 and not status.any()
+    ens.sync()
+    for i, mem in enumerate(ens.members):
+        ref = _single(int(sz[i]), int(b[i]), int(c[i]), float(em[i]), 100, precision=precision)
+        assert mem.get("cycle") == ref.get("cycle") == 100
+        assert mem.get("deltatime") == ref.get("deltatime"), i
+        for f in STATE_FIELDS:
+            assert np.array_equal(mem.real(f), ref.real(f)), (i, f)
+        ref.close()
+    ens.close()
 
 
 def test_members_stop_at_their_own_end():
