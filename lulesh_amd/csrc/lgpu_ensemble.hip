@@ -389,9 +389,10 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
          fprintf(stderr, "lgpu_ensemble_create: members must share precision and constants\n");
          return LGPU_EINVAL;
       }
-      // Validate each member on its own (connectivity ranges, regions).
-      if (d.numElem < 1 || d.numNode < 1 || !d.nodelist || !d.nodeElemStart ||
-          !d.nodeElemCornerList || d.numReg < 1 || d.cost < 0)
+      // Validate each member on its own (connectivity ranges, regions); the
+      // size limit is lgpu_create's (8 corners per element index in int32).
+      if (d.numElem < 1 || d.numNode < 1 || d.numElem > INT32_MAX / 8 || !d.nodelist ||
+          !d.nodeElemStart || !d.nodeElemCornerList || d.numReg < 1 || d.cost < 0)
          return LGPU_EINVAL;
       for (int64_t i = 0; i < 8 * (int64_t)d.numElem; ++i)
          if (d.nodelist[i] < 0 || d.nodelist[i] >= d.numNode) return LGPU_EINVAL;
@@ -444,6 +445,19 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
    if (hipStreamCreateWithFlags(&e->stream, hipStreamNonBlocking) != hipSuccess)
       return bail(LGPU_EDEVICE);
 
+   // The concatenated arrays are indexed with int32 (corners: 8 per element).
+   {
+      int64_t totE = 0, totN = 0;
+      for (int32_t m = 0; m < nMembers; ++m) {
+         totE += members[m].numElem;
+         totN += members[m].numNode;
+      }
+      if (totE > INT32_MAX / 8 || totN > INT32_MAX) {
+         fprintf(stderr, "lgpu_ensemble_create: %lld elements / %lld nodes in all: too many\n",
+                 (long long)totE, (long long)totN);
+         return bail(LGPU_EINVAL);
+      }
+   }
    // Offsets.
    e->nodeStart.assign((size_t)nMembers + 1, 0);
    e->elemStart.assign((size_t)nMembers + 1, 0);

@@ -54,3 +54,17 @@ def test_create_rejects_bad_mesh_without_touching_the_device():
     o.ints("nodelist")[5] = 10_000
     with pytest.raises(RuntimeError, match="EINVAL"):
         context_from_oracle(o)
+
+
+def test_ensemble_create_rejects_oversized_member_without_touching_the_device():
+    """A member beyond lgpu_create's size limit (8 corners per element must
+    index in int32) is refused with EINVAL before any device call."""
+    lib = la.gpu_lib()
+    d = native.MeshDesc()
+    d.precision = la.FP64
+    d.numElem = 2 ** 28 + 1
+    d.numNode = 1
+    out = ctypes.c_void_p()
+    rc = lib.lgpu_ensemble_create(ctypes.byref(d), 1, ctypes.byref(out))
+    assert native.STATUS[rc] == "EINVAL"
+    assert not out.value
