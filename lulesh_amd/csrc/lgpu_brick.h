@@ -128,8 +128,8 @@ inline void set_kin_chunk(Brick &g, int chunk)
 }
 
 // Default element planes per workgroup of the kinematics kernel.  s=192 on
-// 256 CUs: 12 for FP64 (3 workgroups per CU), 14 for the mixed form (4 per
-// CU: 3% faster than 12).
+// 256 CUs: 12 for FP64 (3 workgroups per CU), 28 for the mixed form (4 per
+// CU: 1,008 workgroups, all resident at once; 3% faster than 12).
 inline int pick_kin_chunk(const Brick &g, int slots)
 {
    return pick_chunk((int64_t)g.kinTiles * g.kinTiles, g.s, slots, 12);
