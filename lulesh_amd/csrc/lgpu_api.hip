@@ -404,11 +404,13 @@ int alloc_context(const CtxSpec &sp, lgpu_ctx **out)
    std::vector<Slot> slots;
    DevState &s = c->s;
    auto add = [&](void **dst, size_t bytes) { slots.push_back({dst, align_up(bytes)}); };
+   // node arrays: two doubles of padding, which the kinematics kernel's
+   // pair copies may read past the last node
    double **nodeReal[] = {&s.x, &s.y, &s.z, &s.xd, &s.yd, &s.zd, &s.nodalMass};
-   for (double **p : nodeReal) add((void **)p, sizeof(double) * N);
+   for (double **p : nodeReal) add((void **)p, sizeof(double) * (N + 2));
    if (brick) {
       double **second[] = {&s.xo, &s.yo, &s.zo, &s.xdo, &s.ydo, &s.zdo};
-      for (double **p : second) add((void **)p, sizeof(double) * N);
+      for (double **p : second) add((void **)p, sizeof(double) * (N + 2));
    }
    add((void **)&s.nodeBC, N);
    add((void **)&s.nodeElemStart, sizeof(int32_t) * (N + 1));

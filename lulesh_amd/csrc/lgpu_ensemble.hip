@@ -487,11 +487,13 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
    DevState &s = e->s;
    auto add = [&](void **dst, size_t bytes) { slots.push_back({dst, align_up(bytes)}); };
    double **stateNode[] = {&s.x, &s.y, &s.z, &s.xd, &s.yd, &s.zd};
-   for (double **p : stateNode) add((void **)p, 8 * N);
+   // two doubles of padding, which the kinematics kernel's pair copies may
+   // read past the last node
+   for (double **p : stateNode) add((void **)p, 8 * (N + 2));
    if (e->brick) {
       // the second node buffer is state too: either may be current
       double **second[] = {&s.xo, &s.yo, &s.zo, &s.xdo, &s.ydo, &s.zdo};
-      for (double **p : second) add((void **)p, 8 * N);
+      for (double **p : second) add((void **)p, 8 * (N + 2));
    }
    double **stateElem[] = {&s.e, &s.v};
    for (double **p : stateElem) add((void **)p, 8 * E);

@@ -66,6 +66,9 @@ struct PolicyFp64 {
    // kinematics brick kernel: fetch the next node plane one plane ahead
    // (24 more VGPRs; FP64 would drop from 3 to 2 waves/SIMD and run 8% slower)
    static constexpr bool kKinPrefetch = false;
+   // kinematics brick kernel: node planes copied into LDS by 16-byte LDS DMA
+   // a plane ahead (no registers; 273 -> 260 us at s=192)
+   static constexpr bool kKinDma = true;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
@@ -88,6 +91,7 @@ struct PolicyMixed {
    static constexpr bool kPrefetchNode = true;
    static constexpr bool kOffset32 = true;
    static constexpr bool kKinPrefetch = true; // 118 VGPRs, 4 waves/SIMD: 3% faster
+   static constexpr bool kKinDma = false; // FP32 velocities in LDS: converted on the way
 };
 
 // Element i of a device array through a 32-bit byte offset: the access
