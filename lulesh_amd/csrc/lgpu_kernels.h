@@ -67,7 +67,8 @@ struct PolicyFp64 {
    // (24 more VGPRs; FP64 would drop from 3 to 2 waves/SIMD and run 8% slower)
    static constexpr bool kKinPrefetch = false;
    // kinematics brick kernel: node planes copied into LDS by 16-byte LDS DMA
-   // a plane ahead (no registers; 273 -> 260 us at s=192)
+   // a plane ahead (no registers; 273 -> 260 us at s=192); FP32 velocities
+   // still pass through registers to be converted
    static constexpr bool kKinDma = true;
 };
 
@@ -91,7 +92,7 @@ struct PolicyMixed {
    static constexpr bool kPrefetchNode = true;
    static constexpr bool kOffset32 = true;
    static constexpr bool kKinPrefetch = true; // 118 VGPRs, 4 waves/SIMD: 3% faster
-   static constexpr bool kKinDma = false; // FP32 velocities in LDS: converted on the way
+   static constexpr bool kKinDma = true; // positions by DMA, velocities via registers (FP32)
 };
 
 // Element i of a device array through a 32-bit byte offset: the access
