@@ -63,13 +63,6 @@ struct PolicyFp64 {
    // brick kernel: 32-bit byte offsets from scalar bases (at32) instead of
    // 64-bit per-lane addresses; FP64 runs 1.4% slower with them (more VGPRs)
    static constexpr bool kOffset32 = false;
-   // kinematics brick kernel: fetch the next node plane one plane ahead
-   // (24 more VGPRs; FP64 would drop from 3 to 2 waves/SIMD and run 8% slower)
-   static constexpr bool kKinPrefetch = false;
-   // kinematics brick kernel: node planes copied into LDS by 16-byte LDS DMA
-   // a plane ahead (no registers; 273 -> 260 us at s=192); FP32 velocities
-   // still pass through registers to be converted
-   static constexpr bool kKinDma = true;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
@@ -91,8 +84,6 @@ struct PolicyMixed {
    static constexpr int kNodalWavesMin = 3, kNodalWavesMax = 3;
    static constexpr bool kPrefetchNode = true;
    static constexpr bool kOffset32 = true;
-   static constexpr bool kKinPrefetch = true; // 118 VGPRs, 4 waves/SIMD: 3% faster
-   static constexpr bool kKinDma = true; // positions by DMA, velocities via registers (FP32)
 };
 
 // Element i of a device array through a 32-bit byte offset: the access
