@@ -4,6 +4,8 @@
   lulesh_amd/lib/liblulesh_host.so   C++ host Domain and driver loop
                                      (include/lulesh_host.h)
   lulesh_amd/lib/lulesh2.0-mi355x    command-line driver, reference flags
+  lulesh_amd/lib_asan/               (--sanitize) the host library with
+                                     AddressSanitizer + UBSan, for tests/sanitize.sh
 
 Every HIP source is compiled for gfx950 only.  The FP64 step kernels and the
 context code are built without FMA contraction so they round as the
@@ -97,6 +99,34 @@ def build(force: bool = True, verbose: bool = True) -> None:
               "-llulesh_gpu", "-Wl,-rpath,$ORIGIN"])
 
 
+SAN_FLAGS = ["-O1", "-g", "-fno-omit-frame-pointer", "-fsanitize=address,undefined",
+             "-fno-sanitize-recover=undefined"]
+
+
+def build_sanitized() -> Path:
+    """The host library (C++ Domain, set-up, VTK writer) with AddressSanitizer
+    and UndefinedBehaviorSanitizer, in lulesh_amd/lib_asan next to a link to
+    the ordinary device library (the HIP code is not instrumented).  Run the
+    CPU tests against it with tests/sanitize.sh."""
+    out = PKG / "lib_asan"
+    out.mkdir(exist_ok=True)
+    gpu = out / "liblulesh_gpu.so"
+    if not (LIBDIR / "liblulesh_gpu.so").exists():
+        build(force=False)
+    if gpu.is_symlink() or gpu.exists():
+        gpu.unlink()
+    gpu.symlink_to(LIBDIR / "liblulesh_gpu.so")
+    objs = []
+    for src in HOST_SOURCES:
+        o = OBJDIR / (Path(src).stem + ".asan.o")
+        _run([CXX, "-std=c++17", *SAN_FLAGS, "-fPIC", "-ffp-contract=off", "-Wall", "-Wextra",
+              f"-I{INCLUDE}", f"-I{CSRC}", "-c", str(CSRC / src), "-o", str(o)])
+        objs.append(str(o))
+    _run([CXX, "-shared", "-fPIC", *SAN_FLAGS, "-o", str(out / "liblulesh_host.so"), *objs,
+          f"-L{out}", "-llulesh_gpu", "-Wl,-rpath,$ORIGIN"])
+    return out
+
+
 def clean() -> None:
     shutil.rmtree(OBJDIR, ignore_errors=True)
     shutil.rmtree(LIBDIR, ignore_errors=True)
@@ -105,4 +135,8 @@ def clean() -> None:
 if __name__ == "__main__":
     if "--clean" in sys.argv:
         clean()
-    build(force="--incremental" not in sys.argv)
+    if "--sanitize" in sys.argv:
+        OBJDIR.mkdir(exist_ok=True)
+        build_sanitized()
+    else:
+        build(force="--incremental" not in sys.argv)

@@ -6,12 +6,15 @@ missing, loading raises.
 from __future__ import annotations
 
 import ctypes
+import os
 from pathlib import Path
 
 import numpy as np
 
 PKG = Path(__file__).resolve().parent
-LIBDIR = PKG / "lib"
+# LULESH_LIB_DIR selects another build of the libraries (the sanitizer
+# build of tests/sanitize.sh); the default is the in-tree build.
+LIBDIR = Path(os.environ.get("LULESH_LIB_DIR", PKG / "lib"))
 GPU_SO = LIBDIR / "liblulesh_gpu.so"
 HOST_SO = LIBDIR / "liblulesh_host.so"
 DRIVER = LIBDIR / "lulesh2.0-mi355x"
@@ -521,6 +524,12 @@ class Ensemble:
 
     def sync(self) -> None:
         check(host_lib().lh_ensemble_sync(self.ptr))
+
+    def upload(self, member: int, name: str, values) -> None:
+        """Replaces one field of one member on the device."""
+        a = np.ascontiguousarray(values, np.float64)
+        check(gpu_lib().lgpu_ensemble_upload(self.dev, member, FIELDS.index(name),
+                                             a.ctypes.data_as(_f64p)))
 
     def set_option(self, option: int, value: int) -> None:
         check(gpu_lib().lgpu_ensemble_set_option(self.dev, option, value))

@@ -43,3 +43,24 @@ def max_rel(a: np.ndarray, b: np.ndarray) -> float:
     """max |a-b| / max(|b|, tiny) over the field (scale of the field)."""
     scale = max(float(np.max(np.abs(b))), 1e-300)
     return float(np.max(np.abs(a - b))) / scale
+
+
+def symmetry(e: np.ndarray, nx: int) -> tuple[float, float, float]:
+    """Plane-0 symmetry figures of lulesh-util.cc:197-212 (MaxAbsDiff,
+    TotalAbsDiff, MaxRelDiff) from an element energy array, summed in the
+    reference's loop order, with its IEEE division (x/0 = inf, 0/0 = NaN,
+    which the max comparison skips)."""
+    max_abs = tot = max_rel_ = 0.0
+    with np.errstate(divide="ignore", invalid="ignore"):
+        for j in range(nx):
+            k = np.arange(j + 1, nx)
+            b = e[k * nx + j]
+            ad = np.abs(e[j * nx + k] - b)
+            rd = ad / b
+            for v, r in zip(ad.tolist(), rd.tolist()):
+                tot += v
+                if max_abs < v:
+                    max_abs = v
+                if max_rel_ < r:
+                    max_rel_ = r
+    return max_abs, tot, max_rel_

@@ -16,7 +16,9 @@ import numpy as np
 
 ROOT = Path(__file__).resolve().parent.parent
 ORACLE_DIR = ROOT / "oracle"
-ORACLE_SO = ORACLE_DIR / "build" / "liblulesh_oracle.so"
+# ORACLE_SO_PATH selects another build of the oracle (the sanitizer build of
+# tests/sanitize.sh)
+ORACLE_SO = Path(os.environ.get("ORACLE_SO_PATH", ORACLE_DIR / "build" / "liblulesh_oracle.so"))
 REF_DIR = ORACLE_DIR / "_ref"
 REFDUMP = REF_DIR / "refdump"
 REF_EXE = REF_DIR / "lulesh2.0"

@@ -174,3 +174,39 @@ def test_brick_chunks_do_not_change_results():
     for a, b in zip(*runs):
         for f in STATE_FIELDS:
             assert np.array_equal(a[f], b[f]), f
+
+
+@pytest.mark.parametrize("generic", [False, True], ids=["brick", "generic"])
+def test_failing_member_stops_alone(generic):
+    """One member (s=10) gets an inward velocity of -2 h/dt at node (3,3,3),
+    which inverts element (2,2,2) in its first cycle (as in
+    tests/error_inputs.py): that member reports VolumeError (-1) and stops
+    after that cycle; every other member runs to the cycle limit and equals
+    its own single-problem run bit for bit."""
+    ens = la.Ensemble(**SMALL, flags=la.native.FLAG_GENERIC if generic else 0)
+    bad = 3
+    s = SMALL["sizes"][bad]
+    mem = ens.members[bad]
+    en, h = s + 1, 1.125 / s
+    nd = (3 * en + 3) * en + 3
+    dt = mem.get("deltatime")
+    for f in ("xd", "yd", "zd"):
+        v = mem.real(f).copy()
+        v[nd] = -2.0 * h / dt
+        ens.upload(bad, f, v)
+    st, status = ens.run(20)
+    print(f"failing member: run status {st}, per member {status.tolist()}")
+    assert st == -1
+    assert status[bad] == -1
+    assert int(mem.get("cycle")) == 1
+    ens.sync()
+    for i, m in enumerate(ens.members):
+        if i == bad:
+            continue
+        assert status[i] == 0
+        ref = _single(SMALL["sizes"][i], SMALL["balances"][i], SMALL["costs"][i],
+                      SMALL["emults"][i], 20)
+        assert m.get("cycle") == ref.get("cycle") == 20
+        for f in STATE_FIELDS:
+            assert np.array_equal(m.real(f), ref.real(f)), (i, f)
+    ens.close()

@@ -15,6 +15,7 @@ energy within 1e-10 relative, symmetry diffs of the same order.
 import numpy as np
 import pytest
 
+import error_inputs
 import lulesh_amd as la
 from helpers import STATE_FIELDS, context_from_oracle, max_rel, summary, upload_state
 from oracle_binding import OracleDomain, have_reference_build, read_dump, run_refdump
@@ -293,6 +294,35 @@ def test_volume_error_code():
     st, _, _ = ctx.time_constraints()
     assert st == -1
     assert o.call("lagrange_leapfrog") == -1
+
+
+ERROR_CASES = {
+    # state edits -> the reference's result (lulesh.cc sites in error_inputs.py)
+    "negative_jacobian": ((error_inputs.negative_jacobian,), -1),
+    "vnew_nonpositive": ((error_inputs.vnew_nonpositive,), -1),
+    "qstop": ((error_inputs.tiny_dt, error_inputs.qstop), -2),
+    "negative_jacobian_and_qstop": ((error_inputs.negative_jacobian, error_inputs.qstop), -1),
+    "vnew_nonpositive_and_qstop": ((error_inputs.vnew_nonpositive, error_inputs.qstop), -1),
+}
+
+
+@pytest.mark.parametrize("generic", PATHS)
+@pytest.mark.parametrize("case", list(ERROR_CASES))
+def test_error_sites_and_precedence(case, generic):
+    """Each exit() check of the reference's cycle, and two in one cycle: the
+    device reports the code of the site the reference reaches first, as the
+    oracle does.  The kernels only set flags and finish normally."""
+    edits, want = ERROR_CASES[case]
+    o = error_inputs.prepared(*edits)
+    ctx = context_from_oracle(o, generic=generic)
+    upload_state(ctx, o)
+    dt = o.get("deltatime")
+    print(f"error case {case} generic={generic}", flush=True)
+    assert ctx.lagrange_nodal(dt) == 0       # enqueue only
+    assert ctx.lagrange_elements(dt) == 0
+    st, _, _ = ctx.time_constraints()
+    assert st == want
+    assert o.call("lagrange_leapfrog") == want
 
 
 def test_brick_options_report_and_override():
