@@ -235,13 +235,12 @@ int set_eos_order(lgpu_ctx *c, int k)
 }
 
 // Device array of a field.  `narrow`: stored as FP32 in the mixed form (the
-// Q gradients, p, q, ss, elemMass, arealg and vdov).
+// Q gradients, p, q and ss).
 double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &narrow)
 {
    isElem = f >= LGPU_E;
    narrow = c->precision == LGPU_MIXED &&
-            (f >= LGPU_DELX_XI || f == LGPU_P || f == LGPU_Q || f == LGPU_SS ||
-             f == LGPU_ELEMMASS || f == LGPU_AREALG || f == LGPU_VDOV);
+            (f >= LGPU_DELX_XI || f == LGPU_P || f == LGPU_Q || f == LGPU_SS);
    DevState &s = c->s;
    switch (f) {
    case LGPU_X: return s.x;
@@ -266,10 +265,10 @@ double *real_field_ptr(lgpu_ctx *c, int f, bool &isElem, bool &narrow)
    case LGPU_VOLO: return s.volo;
    case LGPU_VNEW: return s.vnew;
    case LGPU_DELV: return s.delv;
-   case LGPU_VDOV: return static_cast<double *>(s.vdov);
-   case LGPU_AREALG: return static_cast<double *>(s.arealg);
+   case LGPU_VDOV: return s.vdov;
+   case LGPU_AREALG: return s.arealg;
    case LGPU_SS: return static_cast<double *>(s.ss);
-   case LGPU_ELEMMASS: return static_cast<double *>(s.elemMass);
+   case LGPU_ELEMMASS: return s.elemMass;
    case LGPU_DELX_XI: return static_cast<double *>(s.delx_xi);
    case LGPU_DELX_ETA: return static_cast<double *>(s.delx_eta);
    case LGPU_DELX_ZETA: return static_cast<double *>(s.delx_zeta);
@@ -418,9 +417,9 @@ int alloc_context(const CtxSpec &sp, lgpu_ctx **out)
    add((void **)&s.nodelist, sizeof(int32_t) * 8 * E);
    const int32_t **nbr[] = {&s.lxim, &s.lxip, &s.letam, &s.letap, &s.lzetam, &s.lzetap, &s.elemBC};
    for (const int32_t **p : nbr) add((void **)p, sizeof(int32_t) * E);
-   void **pqsArrays[] = {&s.p, &s.q, &s.ss, &s.elemMass, &s.arealg, &s.vdov};
+   void **pqsArrays[] = {&s.p, &s.q, &s.ss};
    for (void **p : pqsArrays) add(p, gradBytes * E);
-   double **elemReal[] = {&s.e, &s.v, &s.volo, &s.vnew, &s.delv};
+   double **elemReal[] = {&s.e, &s.v, &s.volo, &s.vnew, &s.delv, &s.elemMass, &s.arealg, &s.vdov};
    for (double **p : elemReal) add((void **)p, sizeof(double) * E);
    void **grads[] = {&s.delx_xi, &s.delx_eta, &s.delx_zeta, &s.delv_xi, &s.delv_eta, &s.delv_zeta};
    for (void **p : grads) add(p, gradBytes * E);
@@ -656,8 +655,10 @@ int lgpu_create_sedov(const lgpu_sedov_desc *d, lgpu_ctx **out, lgpu_time_state 
       return LGPU_EINVAL;
    };
    if (!d || !ts) return invalid("null argument");
+   // 8 s^3 corner indices must fit int32 (s <= 645); the bound is tested
+   // before any product is formed
+   if (d->size < 1 || d->size > 645) return invalid("size out of range");
    const int64_t sz = d->size, E = sz * sz * sz, N = (sz + 1) * (sz + 1) * (sz + 1);
-   if (sz < 1 || 8 * E > INT32_MAX || N > INT32_MAX) return invalid("size out of range");
    if (d->precision != LGPU_FP64 && d->precision != LGPU_MIXED && d->precision != LGPU_FP64_FMA)
       return invalid("precision");
    if (d->numReg < 1 || d->cost < 0 || d->balance < 0) return invalid("numReg/balance/cost");
@@ -734,7 +735,6 @@ int lgpu_create_sedov(const lgpu_sedov_desc *d, lgpu_ctx **out, lgpu_time_state 
    t.runStart = dStart;
    t.runDest = dDest;
    t.numRuns = d->numRuns;
-   t.narrowMass = d->precision == LGPU_MIXED;
    if (launch_sedov_setup(c->s, t, c->stream) != hipSuccess) return fail(LGPU_EDEVICE);
    if (hipStreamSynchronize(c->stream) != hipSuccess) return fail(LGPU_EDEVICE);
    (void)hipFree(runTables);

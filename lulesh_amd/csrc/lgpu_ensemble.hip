@@ -336,13 +336,12 @@ int build_brick_tables(lgpu_ensemble *e)
 }
 
 // Device array of a field.  `narrow`: stored as FP32 in the mixed form (the
-// Q gradients, p, q, ss, elemMass, arealg and vdov).
+// Q gradients, p, q and ss).
 double *field_base(lgpu_ensemble *e, int f, bool &isElem, bool &narrow)
 {
    isElem = f >= LGPU_E;
    narrow = e->precision == LGPU_MIXED &&
-            (f >= LGPU_DELX_XI || f == LGPU_P || f == LGPU_Q || f == LGPU_SS ||
-             f == LGPU_ELEMMASS || f == LGPU_AREALG || f == LGPU_VDOV);
+            (f >= LGPU_DELX_XI || f == LGPU_P || f == LGPU_Q || f == LGPU_SS);
    DevState &s = e->s;
    switch (f) {
    case LGPU_X: return s.x;
@@ -359,10 +358,10 @@ double *field_base(lgpu_ensemble *e, int f, bool &isElem, bool &narrow)
    case LGPU_VOLO: return s.volo;
    case LGPU_VNEW: return s.vnew;
    case LGPU_DELV: return s.delv;
-   case LGPU_VDOV: return static_cast<double *>(s.vdov);
-   case LGPU_AREALG: return static_cast<double *>(s.arealg);
+   case LGPU_VDOV: return s.vdov;
+   case LGPU_AREALG: return s.arealg;
    case LGPU_SS: return static_cast<double *>(s.ss);
-   case LGPU_ELEMMASS: return static_cast<double *>(s.elemMass);
+   case LGPU_ELEMMASS: return s.elemMass;
    case LGPU_DELX_XI: return static_cast<double *>(s.delx_xi);
    case LGPU_DELX_ETA: return static_cast<double *>(s.delx_eta);
    case LGPU_DELX_ZETA: return static_cast<double *>(s.delx_zeta);
@@ -509,10 +508,8 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
    add((void **)&s.memberDtOut, 16 * (size_t)nMembers);
    const size_t stateSlots = slots.size();
    add((void **)&s.nodalMass, 8 * N);
-   double **elemFix[] = {&s.volo, &s.vnew, &s.delv};
+   double **elemFix[] = {&s.volo, &s.vnew, &s.delv, &s.elemMass, &s.arealg, &s.vdov};
    for (double **p : elemFix) add((void **)p, 8 * E);
-   void **elemFixNarrow[] = {&s.elemMass, &s.arealg, &s.vdov};
-   for (void **p : elemFixNarrow) add(p, gb * E);
    void **grads[] = {&s.delx_xi, &s.delx_eta, &s.delx_zeta, &s.delv_xi, &s.delv_eta, &s.delv_zeta};
    for (void **p : grads) add(p, gb * E);
    add((void **)&s.nodeBC, N);

@@ -15,8 +15,11 @@
 //
 // The Policy template parameter selects the arithmetic of each part:
 // PolicyFp64 keeps everything in FP64 with the reference's association order,
-// PolicyMixed runs force integration, Q gradients/limiter and EOS pressure
-// and sound speed in FP32 on element-local coordinates.
+// PolicyMixed runs force integration (corner forces and their node sums),
+// Q gradients/limiter and EOS pressure and sound speed in FP32 on
+// element-local coordinates; node accelerations, velocities and positions,
+// element volume, delv, strain rate, characteristic length, the energy
+// update and the dt candidates stay FP64.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -50,7 +53,7 @@ struct PolicyFp64 {
    using grad_t = double;   // Q gradients + limiter arithmetic and storage
    using eos_t = double;    // EOS pressure / sound-speed arithmetic
    using nodesum_t = double; // per-node sums of the corner forces
-   using pqs_t = double;     // stored p, q, ss, elemMass, arealg, vdov
+   using pqs_t = double;     // stored p, q, ss
    static constexpr bool kLocalCoords = false;
    // geometry in the compact regrouped forms (different roundings) instead
    // of the reference's association order
@@ -140,10 +143,11 @@ struct DevState {
    const int32_t *nodelist;
    const int32_t *lxim, *lxip, *letam, *letap, *lzetam, *lzetap, *elemBC;
    double *e, *v, *volo;
-   // P::pqs_t (FP64, or FP32 in the mixed form): p, q, ss, elemMass, arealg, vdov
-   void *p, *q, *ss, *elemMass;
+   // P::pqs_t (FP64, or FP32 in the mixed form): p, q, ss
+   void *p, *q, *ss;
+   double *elemMass;
    double *vnew, *delv;
-   void *arealg, *vdov;
+   double *arealg, *vdov;
    double *ql, *qq; // diagnostics (may be null)
    void *delx_xi, *delx_eta, *delx_zeta, *delv_xi, *delv_eta, *delv_zeta; // grad_t
    // corner planes: [(component*2 + kind)*8 + corner][elem], kind 0 stress, 1 hourglass
@@ -153,6 +157,9 @@ struct DevState {
    const int32_t *perm;     // region-major element order
    const int32_t *regStart; // numReg+1 offsets into perm
    const int32_t *regRep;   // EOS repeat count per region
+   // EOS repeat count per element (non-null: k_eos walks the elements in
+   // mesh order and ignores perm, regStart and regRep)
+   const int32_t *elemRep;
    // control
    int32_t *err;            // kNumErrSites flags
    double *dtPartial;       // 2 per k_eos block

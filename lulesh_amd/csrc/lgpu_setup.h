@@ -14,7 +14,6 @@ struct SedovSetup {
    const int32_t *runStart;   // device: first element of each region run, numRuns + 1 entries
    const int32_t *runDest;    // device: work-list position of each run's first element
    int32_t numRuns;
-   bool narrowMass;           // elemMass stored in FP32 (mixed form)
 };
 
 // Fills the mesh, connectivity, boundary, work-list and initial-state arrays

@@ -69,7 +69,6 @@ __global__ void __launch_bounds__(256) k_setup_nodes(DevState s, int sz)
 // (SetupElementConnectivities :539-565), boundary masks
 // (SetupBoundaryConditions :569-673), reference volume and element mass
 // (:161-174, CalcElemVolume lulesh.cc:1273-1356) and v = 1 (:97-100).
-template <typename M>
 __global__ void __launch_bounds__(256) k_setup_elems(DevState s, int sz)
 {
    const int64_t E = s.numElem, en = sz + 1, s2 = (int64_t)sz * sz;
@@ -78,7 +77,6 @@ __global__ void __launch_bounds__(256) k_setup_elems(DevState s, int sz)
    int32_t *letam = const_cast<int32_t *>(s.letam), *letap = const_cast<int32_t *>(s.letap);
    int32_t *lzetam = const_cast<int32_t *>(s.lzetam), *lzetap = const_cast<int32_t *>(s.lzetap);
    int32_t *ebc = const_cast<int32_t *>(s.elemBC);
-   M *mass = static_cast<M *>(s.elemMass);
    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < E;
         e += (int64_t)gridDim.x * blockDim.x) {
       const int a = (int)(e % sz), b = (int)((e / sz) % sz), c = (int)(e / s2);
@@ -102,7 +100,7 @@ __global__ void __launch_bounds__(256) k_setup_elems(DevState s, int sz)
       const double z[8] = {zc[0], zc[0], zc[0], zc[0], zc[1], zc[1], zc[1], zc[1]};
       const double volume = hex_volume<double>(x, y, z);
       s.volo[e] = volume;
-      mass[e] = M(volume);
+      s.elemMass[e] = volume;
       s.v[e] = 1.0;
    }
 }
@@ -159,10 +157,7 @@ unsigned grid_for(int64_t n)
 hipError_t launch_sedov_setup(const DevState &s, const SedovSetup &t, hipStream_t st)
 {
    hipLaunchKernelGGL(k_setup_nodes, dim3(grid_for(s.numNode)), dim3(256), 0, st, s, t.size);
-   if (t.narrowMass)
-      hipLaunchKernelGGL(k_setup_elems<float>, dim3(grid_for(s.numElem)), dim3(256), 0, st, s, t.size);
-   else
-      hipLaunchKernelGGL(k_setup_elems<double>, dim3(grid_for(s.numElem)), dim3(256), 0, st, s, t.size);
+   hipLaunchKernelGGL(k_setup_elems, dim3(grid_for(s.numElem)), dim3(256), 0, st, s, t.size);
    hipLaunchKernelGGL(k_setup_mass, dim3(grid_for(s.numNode)), dim3(256), 0, st, s, t.size);
    hipLaunchKernelGGL(k_setup_perm, dim3(grid_for(s.numElem)), dim3(256), 0, st, s, t.runStart,
                       t.runDest, t.numRuns);

@@ -15,7 +15,7 @@ import numpy as np
 import pytest
 
 import lulesh_amd as la
-from helpers import summary
+from helpers import summary, symmetry
 from oracle_binding import OracleDomain
 
 pytestmark = pytest.mark.gpu
@@ -41,7 +41,7 @@ def test_initial_state_matches_oracle(s, r, b, c, emult, precision):
     d = la.SedovDevice(s, r, b, c, emult, precision=precision)
     for f in INIT_FIELDS:
         want = o.real(f)
-        if precision == la.MIXED and f in ("p", "q", "ss", "elemMass"):
+        if precision == la.MIXED and f in ("p", "q", "ss"):
             want = want.astype(np.float32).astype(np.float64)
         got = d.real(f)
         assert np.array_equal(got, want), f
@@ -98,14 +98,23 @@ def test_full_run_s30_to_stoptime():
 
 
 def test_s192_load_imbalance_200_cycles():
+    """The headline workload (s=192, 21 regions, balance 2, cost 2, 200
+    cycles) from the device-side set-up: the reference's iteration count,
+    final origin energy within 1e-10 and plane-0 symmetry figures
+    (lulesh-util.cc:197-212) within a factor 10 of the reference's."""
     key = (192, 21, 2, 2, 200, 1.0)
     ref = summary(*key)
     d = la.SedovDevice(192, 21, 2, 2)
     assert d.run(200) == 0
     assert d.get("cycle") == ref["cycles"] == 200
-    e0 = d.real("e")[0]
-    print(f"s=192 device set-up: e0 {e0:.17e} (reference {float(ref['e0']):.17e})")
+    e = d.real("e")
+    e0 = e[0]
+    sym = symmetry(e, 192)
+    print(f"s=192 device set-up: e0 {e0:.17e} (reference {float(ref['e0']):.17e}) sym {sym}")
     assert abs(e0 - float(ref["e0"])) <= 1e-10 * float(ref["e0"])
+    for got, name in zip(sym, ("maxabs", "totabs", "maxrel")):
+        want = float(ref[name])
+        assert want / 10 <= got <= want * 10, (name, got, want)
     d.close()
 
 
@@ -142,6 +151,8 @@ def test_invalid_descriptions_are_rejected():
     bad[0] = 4
     assert _create(reg=bad) == -100              # region number above numReg
     assert _create(size=0) == -100
+    assert _create(size=646) == -100            # 8 s^3 corners beyond int32
+    assert _create(size=2**31 - 1) == -100       # no overflow before the range test
     assert _create(precision=7) == -100
     assert _create(emult=0.0) == -100
     zero = ln.copy()
