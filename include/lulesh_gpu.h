@@ -156,6 +156,28 @@ int lgpu_lagrange_elements(lgpu_ctx *ctx, double dt);
  * status (0, -1 or -2); on 0 writes *dtcourant and *dthydro. */
 int lgpu_time_constraints(lgpu_ctx *ctx, double *dtcourant, double *dthydro);
 
+/* CalcTimeConstraintsForElems without a context and without any device
+ * allocation.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  It computes what lgpu_time_constraints
+ * returns -- the Courant (lulesh.cc:2447-2512) and hydro (lulesh.cc:2517-2572)
+ * minima, from 1.0e+20, over elements with vdov != 0 -- from device arrays
+ * the caller holds:
+ *   ss, vdov, arealg   FP64 element fields (device pointers)
+ *   elemlist           numElem element indices (device), the union of the
+ *                      region lists; NULL = elements 0..numElem-1
+ *   qqc, dvovmax       Domain constants (lulesh-init.cc:32, 37)
+ *   dtOut              device pointer to 2 doubles: [0] dtcourant, [1] dthydro
+ *   stream             hipStream_t to enqueue on (NULL: the null stream)
+ * Enqueues only; the caller synchronises the stream before reading dtOut.
+ * Workgroup partials go to a fixed per-GPU array of the library (32 KiB,
+ * 2048 partial pairs; a grid-stride loop covers any numElem >= 0), so calls
+ * that overlap on one GPU race.  Returns 0, LGPU_EINVAL (null array or
+ * numElem < 0) or LGPU_EDEVICE. */
+int lgpu_time_constraints_noalloc(const double *ss, const double *vdov, const double *arealg,
+                                  const int32_t *elemlist, int32_t numElem, double qqc,
+                                  double dvovmax, double *dtOut, void *stream);
+
 /* Time-step controls of Domain (lulesh.h:557-567) used by the device-side
  * loop below. */
 typedef struct lgpu_time_state {

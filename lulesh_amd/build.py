@@ -43,6 +43,7 @@ HIP_SOURCES = {
     "lgpu_step_fp64fma.hip": ["-ffp-contract=fast"],
     "lgpu_ensemble.hip": ["-ffp-contract=off"],
     "lgpu_setup.hip": ["-ffp-contract=off"],
+    "lgpu_noalloc.hip": ["-ffp-contract=off"],
 }
 HOST_SOURCES = ["host/domain.cc", "host/glibc_rand.cc", "host/viz.cc"]
 

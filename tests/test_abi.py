@@ -68,3 +68,13 @@ def test_ensemble_create_rejects_oversized_member_without_touching_the_device():
     rc = lib.lgpu_ensemble_create(ctypes.byref(d), 1, ctypes.byref(out))
     assert native.STATUS[rc] == "EINVAL"
     assert not out.value
+
+
+def test_noalloc_time_constraints_rejects_bad_arguments_without_touching_the_device():
+    """lgpu_time_constraints_noalloc checks its pointers and size on the host
+    before any HIP call: a null array or a negative count is EINVAL."""
+    lib = native.gpu_lib()
+    p = ctypes.c_void_p(256)  # never dereferenced: the checks come first
+    assert lib.lgpu_time_constraints_noalloc(None, p, p, None, 10, 2.0, 0.1, p, None) == -100
+    assert lib.lgpu_time_constraints_noalloc(p, p, p, None, 10, 2.0, 0.1, None, None) == -100
+    assert lib.lgpu_time_constraints_noalloc(p, p, p, None, -1, 2.0, 0.1, p, None) == -100
