@@ -45,9 +45,13 @@ enum lgpu_status {
 
 enum lgpu_precision {
    LGPU_FP64 = 0,  /* every quantity in FP64, arithmetic as the reference */
-   LGPU_MIXED = 1, /* FP32 force integration, Q gradients, EOS pressure and
-                      sound speed; FP64 node integration, volumes, energy
-                      update and dt reductions */
+   LGPU_MIXED = 1, /* FP32 arithmetic: stress and hourglass corner forces
+                      and their node sums, Q gradients and limiter, EOS
+                      pressure and sound speed; FP32 storage: p, q, ss and
+                      the Q gradients.  FP64: node accelerations, velocities
+                      and positions, element volume, vnew, delv, strain rate,
+                      characteristic length, the energy update, the dt
+                      candidates and reductions */
    LGPU_FP64_FMA = 2 /* every quantity in FP64, the reference's association
                         order, with each multiply-add fused (one rounding
                         instead of two); results agree with the reference's
