@@ -337,7 +337,7 @@ int resolve_status(const int32_t *err)
 }
 
 // TimeIncrement for the first cycle of a device-side loop; later cycles'
-// increments are folded into k_dt_reduce (lgpu_time.h).
+// increments are folded into the dt reduction, k_dt_slots (lgpu_time.h).
 __global__ void k_time_increment(DevTime *t, const double *dtOut, const int32_t *err)
 {
    if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -429,7 +429,7 @@ int alloc_context(const CtxSpec &sp, lgpu_ctx **out)
    add((void **)&s.regStart, sizeof(int32_t) * (size_t)(sp.numReg + 1));
    add((void **)&s.regRep, sizeof(int32_t) * (size_t)sp.numReg);
    add((void **)&s.err, sizeof(int32_t) * kNumErrSites);
-   add((void **)&s.dtPartial, sizeof(double) * 2 * eosBlocks);
+   add((void **)&s.dtPartial, sizeof(double) * 2 * kDtSlots);
    add((void **)&s.dtOut, sizeof(double) * 2);
    add((void **)&c->devTime, sizeof(DevTime));
    size_t total = 0;
@@ -445,6 +445,18 @@ int alloc_context(const CtxSpec &sp, lgpu_ctx **out)
    for (const Slot &sl : slots) {
       *sl.dst = c->arena + off;
       off += sl.bytes;
+   }
+   // the dt slot pairs start at the key of 1.0e+20 (k_dt_slots resets them
+   // to it after every read): sign bit set on the value's bits
+   {
+      const double big = 1.0e+20;
+      uint64_t key = 0;
+      std::memcpy(&key, &big, sizeof key);
+      key |= 0x8000000000000000ull;
+      const std::vector<uint64_t> init(2 * (size_t)kDtSlots, key);
+      if (hipMemcpy(s.dtPartial, init.data(), sizeof(uint64_t) * init.size(), hipMemcpyHostToDevice) !=
+          hipSuccess)
+         return fail(LGPU_EDEVICE);
    }
    s.numElem = sp.numElem;
    s.numNode = sp.numNode;
@@ -890,7 +902,7 @@ int lgpu_run(lgpu_ctx *c, lgpu_time_state *ts, int32_t maxCycle)
 
    c->s.dtDevice = &c->devTime->deltatime;
    c->s.active = &c->devTime->active;
-   c->s.time = c->devTime; // k_dt_reduce runs the next cycle's TimeIncrement
+   c->s.time = c->devTime; // k_dt_slots runs the next cycle's TimeIncrement
    // Cycles are enqueued in batches between host checks of the loop flag.
    // No more cycles are enqueued than the cycle limit allows, so only a run
    // that ends at stoptime (or on an error) enqueues no-op cycles.
@@ -976,7 +988,7 @@ int lgpu_set_option(lgpu_ctx *c, int option, int64_t value)
       return LGPU_OK;
    case LGPU_OPT_BRICK_ROWS:
       if (!c->s.isBrick || (value != 8 && value != 16)) return LGPU_EINVAL;
-      c->s.brick.rows = (int32_t)value;
+      set_brick_tile(c->s.brick, 16, (int)value);
       return LGPU_OK;
    case LGPU_OPT_EOS_INTERLEAVE:
       if (value < 0 || value > 1024) return LGPU_EINVAL;

@@ -4,6 +4,7 @@
 #pragma once
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdint>
 #include <vector>
 
@@ -135,6 +136,35 @@ inline int pick_kin_chunk(const Brick &g, int slots)
    return pick_chunk((int64_t)g.kinTiles * g.kinTiles, g.s, slots, 12);
 }
 
+// Fused nodal kernel tiles of w threads by h rows: (w-1) x (h-1) nodes each.
+inline void set_brick_tile(Brick &g, int w, int h)
+{
+   g.tileW = w;
+   g.rows = h;
+   g.tilesX = (g.en + w - 2) / (w - 1);
+   g.tilesY = (g.en + h - 2) / (h - 1);
+}
+
+// The tile of an ensemble member's nodal workgroups (256 threads): the
+// width w (rows 256 / w) that needs the fewest tiles per plane, so that few
+// lanes fall outside a small mesh (s=16: 18 x 14 threads, 2 tiles instead of
+// 4 of 16 x 16); ties go to the width nearest 16.
+inline void pick_member_tile(Brick &g)
+{
+   int bestW = 16;
+   int64_t best = INT64_MAX;
+   for (int w = 4; w <= 64; ++w) {
+      const int h = 256 / w;
+      if (h < 3) break;
+      const int64_t tiles = (int64_t)((g.en + w - 2) / (w - 1)) * ((g.en + h - 2) / (h - 1));
+      if (tiles < best || (tiles == best && std::abs(w - 16) < std::abs(bestW - 16))) {
+         best = tiles;
+         bestW = w;
+      }
+   }
+   set_brick_tile(g, bestW, 256 / bestW);
+}
+
 // Tiling of an s^3 brick with the default chunks (the caller may pick
 // others): 15x15-node tiles of the fused nodal kernel, 16x16-element tiles
 // of the kinematics kernel.
@@ -142,20 +172,16 @@ inline void init_brick(Brick &g, int s)
 {
    g.s = s;
    g.en = s + 1;
-   g.tilesX = g.tilesY = (g.en + 14) / 15;
+   set_brick_tile(g, 16, 16);
    g.divS = make_fastdiv((uint32_t)g.s);
    g.divS2 = make_fastdiv((uint32_t)(g.s * g.s));
    g.kinTiles = (g.s + 15) / 16;
-   g.rows = 16;
    set_brick_chunk(g, 8);
    set_kin_chunk(g, 12);
 }
 
 // Workgroups of one launch of each brick kernel.
-inline int64_t nodal_blocks(const Brick &g)
-{
-   return (int64_t)g.tilesX * ((g.en + g.rows - 2) / (g.rows - 1)) * g.chunks;
-}
+inline int64_t nodal_blocks(const Brick &g) { return (int64_t)g.tilesX * g.tilesY * g.chunks; }
 inline int64_t kin_blocks(const Brick &g) { return (int64_t)g.kinTiles * g.kinTiles * g.kinChunks; }
 
 } // namespace lgpu

@@ -299,6 +299,7 @@ int build_brick_tables(lgpu_ensemble *e)
    for (int m = 0; m < M; ++m) {
       Brick g{};
       init_brick(g, e->memberSize[(size_t)m]);
+      pick_member_tile(g);
       set_brick_chunk(g, e->nodalChunk);
       set_kin_chunk(g, e->kinChunk);
       mb[(size_t)m].g = g;

@@ -11,7 +11,9 @@
 //   k_eos    (region-ordered elements)  qstop check, monotonic-Q limiter,
 //                        EOS with its region repeat count, sound speed,
 //                        volume update, Courant/hydro block minima
-//   k_dt_reduce (one block) Courant/hydro minima over the block partials
+//   k_dt_slots / k_dt_reduce (one block per problem) Courant/hydro minima
+//                        over the k_eos slots (one problem: atomic minima
+//                        into 256 slot pairs) or block partials (ensembles)
 //
 // The Policy template parameter selects the arithmetic of each part:
 // PolicyFp64 keeps everything in FP64 with the reference's association order,
@@ -113,9 +115,12 @@ struct FastDiv {
 // brick, which lets the nodal pass run as one fused kernel.
 struct Brick {
    int32_t s, en;          // elements and nodes per edge
-   int32_t tilesX, tilesY; // 15x15-node column tiles
+   int32_t tilesX, tilesY; // k_nodal_brick column tiles of (tileW-1) x (rows-1) nodes
    int32_t chunkZ, chunks; // node planes per z-chunk, number of chunks
    int32_t rows;           // thread rows per k_nodal_brick workgroup: 16 (256 threads) or 8
+                           // for one problem; up to 256 / tileW in ensembles
+   int32_t tileW;          // threads per k_nodal_brick row: 16 for one problem; chosen per
+                           // ensemble member (4..64)
    FastDiv divS, divS2;    // element index -> (a, b, c)
    // k_kin_brick: 15x15-element column tiles, element planes per z-chunk
    int32_t kinTiles, kinChunk, kinChunks;
@@ -162,11 +167,12 @@ struct DevState {
    const int32_t *elemRep;
    // control
    int32_t *err;            // kNumErrSites flags
-   double *dtPartial;       // 2 per k_eos block
+   double *dtPartial;       // one problem: kDtSlots pairs of min_key values; ensemble: 2 per
+                            // k_eos block
    double *dtOut;           // [0] dtcourant, [1] dthydro
    const double *dtDevice;  // device-side deltatime (null: use argument)
    const int32_t *active;   // device-side loop flag (null: always active)
-   // device-side time state: k_dt_reduce also runs the next cycle's
+   // device-side time state: k_dt_slots also runs the next cycle's
    // TimeIncrement (null: host-driven time stepping)
    struct DevTime *time;
    Brick brick;             // valid when isBrick

@@ -11,6 +11,9 @@
 namespace lgpu {
 
 constexpr int kEosBlock = 256;
+// Slot pairs of the one-problem dt minima (k_eos atomics, k_dt_slots); a
+// power of two
+constexpr int kDtSlots = 256;
 
 inline unsigned eos_blocks(int numElem)
 {
