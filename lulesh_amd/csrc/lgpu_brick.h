@@ -165,6 +165,41 @@ inline void pick_member_tile(Brick &g)
    set_brick_tile(g, bestW, 256 / bestW);
 }
 
+// Kinematics tiles of w x h elements (one per thread of a 256-thread
+// workgroup), kept within the LDS of the 16 x 16 tile: (h+1) node rows of
+// 2 ceil((w+1)/2) values, at most 17 x 18.
+inline bool kin_tile_fits(int w, int h)
+{
+   return w >= 1 && h >= 1 && w * h <= 256 && (h + 1) * 2 * ((w + 2) / 2) <= 17 * 18;
+}
+
+inline void set_kin_tile(Brick &g, int w, int h)
+{
+   g.kinW = w;
+   g.kinH = h;
+   g.kinTilesX = (g.s + w - 1) / w;
+   g.kinTilesY = (g.s + h - 1) / h;
+}
+
+// An ensemble member's kinematics tile: the width (rows 256 / width) with
+// the fewest tiles per plane that fits the LDS; ties go to the width
+// nearest 16.
+inline void pick_member_kin_tile(Brick &g)
+{
+   int bestW = 16;
+   int64_t best = INT64_MAX;
+   for (int w = 4; w <= 64; ++w) {
+      const int h = 256 / w;
+      if (!kin_tile_fits(w, h)) continue;
+      const int64_t tiles = (int64_t)((g.s + w - 1) / w) * ((g.s + h - 1) / h);
+      if (tiles < best || (tiles == best && std::abs(w - 16) < std::abs(bestW - 16))) {
+         best = tiles;
+         bestW = w;
+      }
+   }
+   set_kin_tile(g, bestW, 256 / bestW);
+}
+
 // Tiling of an s^3 brick with the default chunks (the caller may pick
 // others): 15x15-node tiles of the fused nodal kernel, 16x16-element tiles
 // of the kinematics kernel.
@@ -176,12 +211,13 @@ inline void init_brick(Brick &g, int s)
    g.divS = make_fastdiv((uint32_t)g.s);
    g.divS2 = make_fastdiv((uint32_t)(g.s * g.s));
    g.kinTiles = (g.s + 15) / 16;
+   set_kin_tile(g, 16, 16);
    set_brick_chunk(g, 8);
    set_kin_chunk(g, 12);
 }
 
 // Workgroups of one launch of each brick kernel.
 inline int64_t nodal_blocks(const Brick &g) { return (int64_t)g.tilesX * g.tilesY * g.chunks; }
-inline int64_t kin_blocks(const Brick &g) { return (int64_t)g.kinTiles * g.kinTiles * g.kinChunks; }
+inline int64_t kin_blocks(const Brick &g) { return (int64_t)g.kinTilesX * g.kinTilesY * g.kinChunks; }
 
 } // namespace lgpu

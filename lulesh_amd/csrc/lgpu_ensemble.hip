@@ -300,6 +300,7 @@ int build_brick_tables(lgpu_ensemble *e)
       Brick g{};
       init_brick(g, e->memberSize[(size_t)m]);
       pick_member_tile(g);
+      pick_member_kin_tile(g);
       set_brick_chunk(g, e->nodalChunk);
       set_kin_chunk(g, e->kinChunk);
       mb[(size_t)m].g = g;
@@ -874,7 +875,7 @@ int lgpu_ensemble_autotune(lgpu_ensemble *e, const lgpu_time_state *ts, int32_t 
    int r = LGPU_OK;
    if (e->brick) {
       // Chunk of each fused kernel over a few candidates, then the XCD mapping.
-      const int nodalCands[4] = {4, 8, 13, 22}, kinCands[4] = {4, 8, 12, 16};
+      const int nodalCands[] = {4, 6, 8, 11, 16, 22}, kinCands[] = {4, 6, 8, 12, 16};
       float best = 1e30f;
       int bestC = e->nodalChunk;
       for (int c : nodalCands) {

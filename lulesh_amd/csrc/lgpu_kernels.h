@@ -122,8 +122,10 @@ struct Brick {
    int32_t tileW;          // threads per k_nodal_brick row: 16 for one problem; chosen per
                            // ensemble member (4..64)
    FastDiv divS, divS2;    // element index -> (a, b, c)
-   // k_kin_brick: 15x15-element column tiles, element planes per z-chunk
+   // k_kin_brick: 16x16-element column tiles per edge, element planes per z-chunk
    int32_t kinTiles, kinChunk, kinChunks;
+   // ensemble members: kinW x kinH element tiles, kinTilesX x kinTilesY per plane
+   int32_t kinW, kinH, kinTilesX, kinTilesY;
 };
 
 // One ensemble member on the brick path: its brick, where its nodes and
