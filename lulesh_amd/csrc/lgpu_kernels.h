@@ -68,11 +68,17 @@ struct PolicyFp64 {
    // brick kernel: 32-bit byte offsets from scalar bases (at32) instead of
    // 64-bit per-lane addresses; FP64 runs 1.4% slower with them (more VGPRs)
    static constexpr bool kOffset32 = false;
+   // brick kernel: issue the first hourglass velocity gather before the
+   // hourglass modes (true) or after them; the later components are always
+   // loaded one component ahead.  FP64 s=192: after, 598 us against 748 (256
+   // VGPRs before); FMA and mixed: before, 1.4% and 0.5% faster
+   static constexpr bool kHgVelEarly = false;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
 // translation unit); the types are those of PolicyFp64.
 struct PolicyFp64Fma : PolicyFp64 {
+   static constexpr bool kHgVelEarly = true;
 };
 
 struct PolicyMixed {
@@ -89,6 +95,7 @@ struct PolicyMixed {
    static constexpr int kNodalWavesMin = 3, kNodalWavesMax = 3;
    static constexpr bool kPrefetchNode = true;
    static constexpr bool kOffset32 = true;
+   static constexpr bool kHgVelEarly = true;
 };
 
 // Element i of a device array through a 32-bit byte offset: the access
