@@ -73,6 +73,9 @@ struct PolicyFp64 {
    // loaded one component ahead.  FP64 s=192: after, 598 us against 748 (256
    // VGPRs before); FMA and mixed: before, 1.4% and 0.5% faster
    static constexpr bool kHgVelEarly = false;
+   // k_kin_brick: load volo and v a plane ahead.  Mixed s=192: 190 -> 181 us;
+   // FP64: 249 -> 269 us (the extra registers cost it its third wave)
+   static constexpr bool kKinPrefetch = false;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
@@ -96,6 +99,7 @@ struct PolicyMixed {
    static constexpr bool kPrefetchNode = true;
    static constexpr bool kOffset32 = true;
    static constexpr bool kHgVelEarly = true;
+   static constexpr bool kKinPrefetch = true;
 };
 
 // Element i of a device array through a 32-bit byte offset: the access
