@@ -53,6 +53,7 @@ struct lgpu_ctx {
    int block = 256;
    bool keepDiag = false;
    std::vector<int32_t> eosBlockRep; // repeat count of each k_eos block (region order)
+   std::vector<int32_t> hostRegStart, hostRegRep; // the work-list groups (host copies)
    int eosInterleave = 0;
    int32_t brickPerCu = 0;
    bool timing = false;
@@ -227,9 +228,37 @@ int set_eos_order(lgpu_ctx *c, int k)
          for (int j = 0; j < k && r < rest.size(); ++j) order.push_back(rest[r++]);
       }
    }
+   // the repeat counts of each block from the group tables: its first
+   // position's group, and the next group where the block reaches into it
+   const std::vector<int32_t> &rs = c->hostRegStart, &rr = c->hostRegRep;
+   const int ng = (int)rr.size();
+   auto group_of = [&](int64_t p) {
+      int g = (int)(std::upper_bound(rs.begin(), rs.begin() + ng, (int32_t)p) - rs.begin()) - 1;
+      while (g + 1 < ng && rs[(size_t)g + 1] == rs[(size_t)g]) ++g; // skip empty groups
+      return std::max(0, g);
+   };
+   std::vector<EosSlot> slots(nb);
+   for (size_t i = 0; i < nb; ++i) {
+      const int64_t p0 = (int64_t)order[i] * kEosBlock;
+      const int64_t p1 = std::min<int64_t>(p0 + kEosBlock, c->numElem) - 1;
+      EosSlot &sl = slots[i];
+      sl.block = order[i];
+      const int g0 = group_of(p0), g1 = group_of(std::max(p0, p1));
+      if (g0 == g1) {
+         sl.rep0 = sl.rep1 = rr[(size_t)g0];
+         sl.split = (int32_t)(p0 + kEosBlock);
+      } else if (group_of(rs[(size_t)g0 + 1]) == g1) {
+         sl.rep0 = rr[(size_t)g0];
+         sl.rep1 = rr[(size_t)g1];
+         sl.split = rs[(size_t)g0 + 1];
+      } else {
+         sl.rep0 = -1; // three or more groups: k_eos searches regStart
+         sl.split = sl.rep1 = 0;
+      }
+   }
    HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
-   HIP_OR_RETURN(c, hipMemcpy(const_cast<int32_t *>(c->s.eosBlockOrder), order.data(),
-                              sizeof(int32_t) * nb, hipMemcpyHostToDevice));
+   HIP_OR_RETURN(c, hipMemcpy(const_cast<EosSlot *>(c->s.eosSlot), slots.data(),
+                              sizeof(EosSlot) * nb, hipMemcpyHostToDevice));
    c->eosInterleave = std::max(0, k);
    return LGPU_OK;
 }
@@ -425,7 +454,7 @@ int alloc_context(const CtxSpec &sp, lgpu_ctx **out)
    for (void **p : grads) add(p, gradBytes * E);
    if (!brick) add(&s.corner, cornerBytes * (size_t)plane * 48);
    add((void **)&s.perm, sizeof(int32_t) * E);
-   add((void **)&s.eosBlockOrder, sizeof(int32_t) * eosBlocks);
+   add((void **)&s.eosSlot, sizeof(EosSlot) * eosBlocks);
    add((void **)&s.regStart, sizeof(int32_t) * (size_t)(sp.numReg + 1));
    add((void **)&s.regRep, sizeof(int32_t) * (size_t)sp.numReg);
    add((void **)&s.err, sizeof(int32_t) * kNumErrSites);
@@ -521,6 +550,8 @@ int set_groups(lgpu_ctx *c, const std::vector<int32_t> &reps, const std::vector<
          const size_t b = (size_t)((regStart[(size_t)i + 1] - 1) / kEosBlock);
          c->eosBlockRep[b] = std::max(c->eosBlockRep[b], regRep[(size_t)i]);
       }
+   c->hostRegStart = regStart;
+   c->hostRegRep = regRep;
    HIP_OR_RETURN(c, hipMemcpy(const_cast<int32_t *>(s.regStart), regStart.data(),
                               sizeof(int32_t) * regStart.size(), hipMemcpyHostToDevice));
    HIP_OR_RETURN(c, hipMemcpy(const_cast<int32_t *>(s.regRep), regRep.data(),

@@ -43,6 +43,12 @@ enum ErrSite {
    kNumErrSites = 8
 };
 
+// One k_eos workgroup's perm block and its repeat counts (a block spans at
+// most two work-list groups unless a group holds fewer than a block).
+struct EosSlot {
+   int32_t block, rep0, split, rep1;
+};
+
 struct Consts {
    double e_cut, p_cut, q_cut, v_cut, u_cut;
    double hgcoef, ss4o3, qstop, monoq_max_slope, monoq_limiter_mult;
@@ -175,9 +181,6 @@ struct DevState {
    const int32_t *perm;     // region-major element order
    const int32_t *regStart; // numReg+1 offsets into perm
    const int32_t *regRep;   // EOS repeat count per region
-   // EOS repeat count per element (non-null: k_eos walks the elements in
-   // mesh order and ignores perm, regStart and regRep)
-   const int32_t *elemRep;
    // control
    int32_t *err;            // kNumErrSites flags
    double *dtPartial;       // one problem: kDtSlots pairs of min_key values; ensemble: 2 per
@@ -191,9 +194,11 @@ struct DevState {
    Brick brick;             // valid when isBrick
    int32_t isBrick;
    int32_t brickFaces; // k_eos computes face neighbours and BC masks from the brick
-   // k_eos dispatch order: workgroup i processes perm block eosBlockOrder[i]
-   // (null: block i)
-   const int32_t *eosBlockOrder;
+   // k_eos dispatch table (one problem): workgroup i processes perm block
+   // eosSlot[i].block, whose positions below eosSlot[i].split take repeat
+   // count rep0 and the others rep1 (rep0 < 0: look each position up in
+   // regStart).  Null (ensembles): block i, counts looked up.
+   const struct EosSlot *eosSlot;
    // Ensemble of independent problems (nMembers > 0): members are
    // concatenated; each has its own dt, loop flag and error flags.
    int32_t nMembers;
