@@ -228,34 +228,7 @@ int set_eos_order(lgpu_ctx *c, int k)
          for (int j = 0; j < k && r < rest.size(); ++j) order.push_back(rest[r++]);
       }
    }
-   // the repeat counts of each block from the group tables: its first
-   // position's group, and the next group where the block reaches into it
-   const std::vector<int32_t> &rs = c->hostRegStart, &rr = c->hostRegRep;
-   const int ng = (int)rr.size();
-   auto group_of = [&](int64_t p) {
-      int g = (int)(std::upper_bound(rs.begin(), rs.begin() + ng, (int32_t)p) - rs.begin()) - 1;
-      while (g + 1 < ng && rs[(size_t)g + 1] == rs[(size_t)g]) ++g; // skip empty groups
-      return std::max(0, g);
-   };
-   std::vector<EosSlot> slots(nb);
-   for (size_t i = 0; i < nb; ++i) {
-      const int64_t p0 = (int64_t)order[i] * kEosBlock;
-      const int64_t p1 = std::min<int64_t>(p0 + kEosBlock, c->numElem) - 1;
-      EosSlot &sl = slots[i];
-      sl.block = order[i];
-      const int g0 = group_of(p0), g1 = group_of(std::max(p0, p1));
-      if (g0 == g1) {
-         sl.rep0 = sl.rep1 = rr[(size_t)g0];
-         sl.split = (int32_t)(p0 + kEosBlock);
-      } else if (group_of(rs[(size_t)g0 + 1]) == g1) {
-         sl.rep0 = rr[(size_t)g0];
-         sl.rep1 = rr[(size_t)g1];
-         sl.split = rs[(size_t)g0 + 1];
-      } else {
-         sl.rep0 = -1; // three or more groups: k_eos searches regStart
-         sl.split = sl.rep1 = 0;
-      }
-   }
+   const std::vector<EosSlot> slots = eos_slots(order, c->hostRegStart, c->hostRegRep, c->numElem);
    HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
    HIP_OR_RETURN(c, hipMemcpy(const_cast<EosSlot *>(c->s.eosSlot), slots.data(),
                               sizeof(EosSlot) * nb, hipMemcpyHostToDevice));

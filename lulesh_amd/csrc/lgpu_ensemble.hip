@@ -530,6 +530,7 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
    add((void **)&s.memberNodeStart, 4 * ((size_t)nMembers + 1));
    add((void **)&s.memberElemStart, 4 * ((size_t)nMembers + 1));
    add((void **)&s.blockMember, 4 * (size_t)blockStart[nMembers]);
+   add((void **)&s.eosSlot, sizeof(EosSlot) * (size_t)blockStart[nMembers]);
    add((void **)&s.memberBlockStart, 4 * ((size_t)nMembers + 1));
    size_t total = 0, stateBytes = 0;
    for (size_t i = 0; i < slots.size(); ++i) {
@@ -604,6 +605,10 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
       for (int32_t b = blockStart[m]; b < blockStart[m + 1]; ++b) blockMember[(size_t)b] = m;
    }
    regStart[(size_t)numRegTot] = (int32_t)nPerm;
+   // k_eos dispatch slots: blocks in order, each with its repeat counts
+   std::vector<int32_t> blockOrder((size_t)blockStart[nMembers]);
+   for (size_t b = 0; b < blockOrder.size(); ++b) blockOrder[b] = (int32_t)b;
+   const std::vector<EosSlot> eosSlots = eos_slots(blockOrder, regStart, regRep, (int64_t)nPerm);
 
    auto h2d = [&](void *dst, const void *src, size_t bytes) {
       return hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice) == hipSuccess;
@@ -623,6 +628,7 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
              h2d((void *)s.memberNodeStart, e->nodeStart.data(), 4 * e->nodeStart.size()) &&
              h2d((void *)s.memberElemStart, e->elemStart.data(), 4 * e->elemStart.size()) &&
              h2d((void *)s.blockMember, blockMember.data(), 4 * blockMember.size()) &&
+             h2d((void *)s.eosSlot, eosSlots.data(), sizeof(EosSlot) * eosSlots.size()) &&
              h2d((void *)s.memberBlockStart, blockStart.data(), 4 * blockStart.size()) &&
              h2d(s.v, ones.data(), 8 * E);
    if (!ok) return bail(LGPU_EDEVICE);
