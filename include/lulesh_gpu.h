@@ -182,6 +182,16 @@ int lgpu_time_constraints_noalloc(const double *ss, const double *vdov, const do
                                   const int32_t *elemlist, int32_t numElem, double qqc,
                                   double dvovmax, double *dtOut, void *stream);
 
+/* Diagnostics; no counterpart in the reference.  Evaluates the FP64
+ * reciprocal and square root that k_eos uses inside its repeats on n device
+ * values x, writing 1/x[i] to recip and sqrt(x[i]) to root (device arrays;
+ * either may be NULL), enqueued on stream.  They are correctly rounded for
+ * 2^-400 <= |x| <= 2^400 (reciprocal) and 2^-767 <= x < +inf (square root);
+ * outside those ranges k_eos uses the general routines and so may this
+ * call's results differ from 1/x and sqrt(x).  Returns 0, LGPU_EINVAL (null
+ * x or n < 0) or LGPU_EDEVICE. */
+int lgpu_fp64_routines(const double *x, int64_t n, double *recip, double *root, void *stream);
+
 /* Time-step controls of Domain (lulesh.h:557-567) used by the device-side
  * loop below. */
 typedef struct lgpu_time_state {

@@ -549,9 +549,34 @@ int finish_context(lgpu_ctx *c)
    return LGPU_OK;
 }
 
+// The EOS's FP64 reciprocal and square root on caller data (lgpu_fp64_routines).
+__global__ void __launch_bounds__(256)
+k_fp64_routines(const double *__restrict__ x, int64_t n, double *__restrict__ rc,
+                double *__restrict__ rt)
+{
+   const int64_t stride = (int64_t)gridDim.x * 256;
+   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += stride) {
+      const double v = x[i];
+      if (rc) rc[i] = recip_rn_normal(v);
+      if (rt) rt[i] = sqrt_rn_normal(v);
+   }
+}
+
 } // namespace
 
 extern "C" {
+
+int lgpu_fp64_routines(const double *x, int64_t n, double *recip, double *root, void *stream)
+{
+   if (!x || n < 0) return LGPU_EINVAL;
+   if (n == 0) return LGPU_OK;
+   const int64_t want = (n + 255) / 256;
+   const int grid = (int)(want < 4096 ? want : 4096);
+   hipLaunchKernelGGL(k_fp64_routines, dim3(grid), dim3(256), 0, static_cast<hipStream_t>(stream),
+                      x, n, recip, root);
+   if (hipGetLastError() != hipSuccess) return LGPU_EDEVICE;
+   return LGPU_OK;
+}
 
 void lgpu_default_constants(lgpu_constants *k)
 {

@@ -26,6 +26,38 @@ LG_DEV float root(float x) { return __builtin_amdgcn_sqrtf(x); }
 LG_DEV double div_root(double a, double x) { return a / sqrt(x); }
 LG_DEV float div_root(float a, float x) { return a * __builtin_amdgcn_rsqf(x); }
 
+// Correctly rounded FP64 reciprocal and square root for arguments that need
+// none of the general routines' range handling.  Both refine the hardware
+// estimate (v_rcp_f64 / v_rsq_f64) by Newton-Raphson steps and finish with
+// one FMA correction of the rounded result (Markstein, IBM J. Res. Dev.
+// 34(1), 1990): the steps of the general routines, without their exponent
+// scaling and special-value fix-ups, so inside the stated ranges the results
+// are the correctly rounded ones, bit for bit those of 1.0 / x and sqrt(x).
+//   recip_rn_normal: 2^-400 <= |x| <= 2^400
+//   sqrt_rn_normal:  2^-767 <= x < +inf
+LG_DEV double recip_rn_normal(double x)
+{
+   double r = __builtin_amdgcn_rcp(x);
+   double e = __builtin_fma(-x, r, 1.0);
+   r = __builtin_fma(r, e, r);
+   e = __builtin_fma(-x, r, 1.0);
+   r = __builtin_fma(r, e, r);
+   e = __builtin_fma(-x, r, 1.0);
+   return __builtin_fma(e, r, r);
+}
+LG_DEV double sqrt_rn_normal(double x)
+{
+   const double y = __builtin_amdgcn_rsq(x);
+   double g = x * y, h = y * 0.5;
+   const double r = __builtin_fma(-h, g, 0.5);
+   g = __builtin_fma(g, r, g);
+   h = __builtin_fma(h, r, h);
+   double d = __builtin_fma(-g, g, x);
+   g = __builtin_fma(d, h, g);
+   d = __builtin_fma(-g, g, x);
+   return __builtin_fma(d, h, g);
+}
+
 // u . (v x w); the triple product of lulesh.cc:1337-1338 as it is applied
 // at lulesh.cc:1341-1349.
 template <typename T>

@@ -116,6 +116,8 @@ def gpu_lib() -> ctypes.CDLL:
         lib.lgpu_time_constraints.restype = ctypes.c_int
         lib.lgpu_time_constraints_noalloc.argtypes = [vp, vp, vp, vp, i32, dbl, dbl, vp, vp]
         lib.lgpu_time_constraints_noalloc.restype = ctypes.c_int
+        lib.lgpu_fp64_routines.argtypes = [vp, ctypes.c_int64, vp, vp, vp]
+        lib.lgpu_fp64_routines.restype = ctypes.c_int
         lib.lgpu_run.argtypes = [vp, ctypes.POINTER(TimeState), i32]
         lib.lgpu_run.restype = ctypes.c_int
         lib.lgpu_set_option.argtypes = [vp, ctypes.c_int, ctypes.c_int64]

@@ -15,7 +15,7 @@ ROOT = Path(__file__).resolve().parent.parent
 def declared(header: str) -> list[str]:
     text = (ROOT / "include" / header).read_text()
     text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
-    names = re.findall(r"\b(l(?:gpu|h)_[a-z_]+)\s*\(", text)
+    names = re.findall(r"\b(l(?:gpu|h)_[a-z0-9_]+)\s*\(", text)
     return sorted(set(names))
 
 
@@ -78,3 +78,13 @@ def test_noalloc_time_constraints_rejects_bad_arguments_without_touching_the_dev
     assert lib.lgpu_time_constraints_noalloc(None, p, p, None, 10, 2.0, 0.1, p, None) == -100
     assert lib.lgpu_time_constraints_noalloc(p, p, p, None, 10, 2.0, 0.1, None, None) == -100
     assert lib.lgpu_time_constraints_noalloc(p, p, p, None, -1, 2.0, 0.1, p, None) == -100
+
+
+def test_fp64_routines_reject_bad_arguments_without_touching_the_device():
+    """lgpu_fp64_routines checks its input pointer and count on the host, and
+    n == 0 enqueues nothing."""
+    lib = native.gpu_lib()
+    p = ctypes.c_void_p(16)
+    assert lib.lgpu_fp64_routines(None, 10, p, p, None) == -100
+    assert lib.lgpu_fp64_routines(p, -1, p, p, None) == -100
+    assert lib.lgpu_fp64_routines(p, 0, p, p, None) == 0

@@ -14,6 +14,8 @@ import pytest
 import lulesh_amd as la
 from lulesh_amd import native
 
+from hip_buffers import HipBuffers
+
 pytestmark = pytest.mark.gpu
 
 
@@ -38,43 +40,7 @@ def _state_after(s: int, cycles: int):
     return want, fields, lists
 
 
-class _HipBuffers:
-    """Device buffers of the test (the caller's inputs and outputs), from the
-    HIP runtime the library itself is linked against.  Torch is not used:
-    its wheel carries a second HIP runtime, which cannot open the GPU once
-    the library's runtime holds it in the same process."""
-
-    def __init__(self):
-        native.gpu_lib()  # loads the library's runtime first; the soname below resolves to it
-        self.hip = ctypes.CDLL("libamdhip64.so.7")
-        self.hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
-        self.hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
-        self.hip.hipFree.argtypes = [ctypes.c_void_p]
-        self.hip.hipDeviceSynchronize.argtypes = []
-        self.ptrs = []
-
-    def put(self, a: np.ndarray) -> ctypes.c_void_p:
-        p = ctypes.c_void_p()
-        assert self.hip.hipMalloc(ctypes.byref(p), max(a.nbytes, 16)) == 0
-        self.ptrs.append(p)
-        assert self.hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
-        return p
-
-    def get(self, p: ctypes.c_void_p, n: int, dtype) -> np.ndarray:
-        out = np.empty(n, dtype)
-        assert self.hip.hipMemcpy(out.ctypes.data, p, out.nbytes, 2) == 0  # device to host
-        return out
-
-    def sync(self):
-        assert self.hip.hipDeviceSynchronize() == 0
-
-    def free(self):
-        for p in self.ptrs:
-            self.hip.hipFree(p)
-        self.ptrs = []
-
-
-def _noalloc(bufs: _HipBuffers, fields, lists, use_list: bool):
+def _noalloc(bufs: HipBuffers, fields, lists, use_list: bool):
     ss, vdov, ar = (bufs.put(fields[f]) for f in ("ss", "vdov", "arealg"))
     lst = bufs.put(lists) if use_list else None
     out = bufs.put(np.full(2, -1.0))
@@ -94,7 +60,7 @@ def test_noalloc_time_constraints_equal_context_entry_point():
     big = _state_after(24, 12)
     small = _state_after(6, 5)
     assert len(big[2]) == 24 ** 3 and len(small[2]) == 6 ** 3
-    bufs = _HipBuffers()
+    bufs = HipBuffers()
     # the larger problem first, over all elements 0..n-1
     got = _noalloc(bufs, big[1], big[2], use_list=False)
     print("noalloc s=24:", got, "context:", big[0], flush=True)
