@@ -82,6 +82,11 @@ struct PolicyFp64 {
    // k_kin_brick: load volo and v a plane ahead.  Mixed s=192: 190 -> 181 us;
    // FP64: 249 -> 269 us (the extra registers cost it its third wave)
    static constexpr bool kKinPrefetch = false;
+   // brick kernel: a scheduling barrier after the element-field loads (p, q,
+   // v, volo, ss, elemMass), so that none is sunk to its use.  Mixed s=192:
+   // 416 -> 411 us; FP64 582 -> 600 us, FMA 505 -> 534 us (the stress work
+   // then waits for every load at once)
+   static constexpr bool kPinElemFields = false;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
@@ -106,6 +111,7 @@ struct PolicyMixed {
    static constexpr bool kOffset32 = true;
    static constexpr bool kHgVelEarly = true;
    static constexpr bool kKinPrefetch = true;
+   static constexpr bool kPinElemFields = true;
 };
 
 // Element i of a device array through a 32-bit byte offset: the access
