@@ -87,43 +87,55 @@ LG_DEV T hex_volume(const T (&x)[8], const T (&y)[8], const T (&z)[8])
    return v;
 }
 
-// Centre Jacobian of the trilinear map, one physical coordinate (the three
-// entries of lulesh.cc:319-329 for that coordinate).
+// Scaled forms.  Several reference formulas multiply by a power of two
+// (.125, .5, .25, 8) before further products and sums.  Such a factor is
+// exact, and rounding commutes with it: RN(2^k a) = 2^k RN(a), and so for
+// products, sums and quotients of scaled operands.  The *N functions below
+// leave these factors out and return N times the reference's quantity, bit
+// for bit N times it as long as no intermediate value leaves the normal
+// range (subnormal or infinite, which no mesh of sensible extent comes near);
+// their callers put the factor back in one place (or nowhere, where it
+// cancels or only a sign is used).
+
+// Centre Jacobian of the trilinear map, one physical coordinate: 8 times the
+// three entries of lulesh.cc:319-329 for that coordinate (without their .125).
 template <typename T>
 struct JacRow {
    T xi, eta, zeta;
 };
 
 template <typename T>
-LG_DEV JacRow<T> jac_row(const T (&c)[8])
+LG_DEV JacRow<T> jac_row8(const T (&c)[8])
 {
    const T a = c[6] - c[0], b = c[5] - c[3], g = c[7] - c[1], h = c[4] - c[2];
    JacRow<T> r;
-   r.xi = T(.125) * (a + b - g - h);
-   r.eta = T(.125) * (a - b + g - h);
-   r.zeta = T(.125) * (a + b + g + h);
+   r.xi = a + b - g - h;
+   r.eta = a - b + g - h;
+   r.zeta = a + b + g + h;
    return r;
 }
 
-// Jacobian determinant ("volume") of CalcElemShapeFunctionDerivatives,
-// lulesh.cc:332-376, without the derivative rows (IntegrateStressForElems
-// overwrites them with face normals, lulesh.cc:534-538).
+// 64 times the Jacobian determinant ("volume") of
+// CalcElemShapeFunctionDerivatives, lulesh.cc:332-376, without the
+// derivative rows (IntegrateStressForElems overwrites them with face normals,
+// lulesh.cc:534-538, and uses only the determinant's sign).
 template <typename T>
-LG_DEV T jacobian_volume(const T (&x)[8], const T (&y)[8], const T (&z)[8])
+LG_DEV T jacobian_volume64(const T (&x)[8], const T (&y)[8], const T (&z)[8])
 {
-   const JacRow<T> jx = jac_row(x), jy = jac_row(y), jz = jac_row(z);
+   const JacRow<T> jx = jac_row8(x), jy = jac_row8(y), jz = jac_row8(z);
    const T cxet = (jz.xi * jy.zeta) - (jy.xi * jz.zeta);
    const T cyet = (jx.xi * jz.zeta) - (jz.xi * jx.zeta);
    const T czet = (jy.xi * jx.zeta) - (jx.xi * jy.zeta);
-   return T(8.) * (jx.eta * cxet + jy.eta * cyet + jz.eta * czet);
+   return jx.eta * cxet + jy.eta * cyet + jz.eta * czet;
 }
 
 // Shape-function derivatives of nodes 0..3 (4..7 are their negatives,
-// lulesh.cc:344-373) and the Jacobian determinant, lulesh.cc:290-377.
+// lulesh.cc:344-373) and the Jacobian determinant, lulesh.cc:290-377, both
+// 64 times the reference's (the quotient of the two is the reference's).
 template <typename T>
-LG_DEV T shape_derivatives(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&b)[3][4])
+LG_DEV T shape_derivatives64(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&b)[3][4])
 {
-   const JacRow<T> jx = jac_row(x), jy = jac_row(y), jz = jac_row(z);
+   const JacRow<T> jx = jac_row8(x), jy = jac_row8(y), jz = jac_row8(z);
    const T c[3][3] = {
       {(jy.eta * jz.zeta) - (jz.eta * jy.zeta), (jz.xi * jy.zeta) - (jy.xi * jz.zeta),
        (jy.xi * jz.eta) - (jz.xi * jy.eta)},
@@ -138,43 +150,44 @@ LG_DEV T shape_derivatives(const T (&x)[8], const T (&y)[8], const T (&z)[8], T 
       b[r][2] = c[r][0] + c[r][1] - c[r][2];
       b[r][3] = -c[r][0] + c[r][1] - c[r][2];
    }
-   return T(8.) * (jx.eta * c[0][1] + jy.eta * c[1][1] + jz.eta * c[2][1]);
+   return jx.eta * c[0][1] + jy.eta * c[1][1] + jz.eta * c[2][1];
 }
 
-// Area-weighted normal of one quadrilateral face (n0,n1,n2,n3), added to its
-// four nodes: SumElemFaceNormal, lulesh.cc:381-415.
+// 16 times the area-weighted normal of one quadrilateral face (n0,n1,n2,n3),
+// added to its four nodes: SumElemFaceNormal, lulesh.cc:381-415, without the
+// .5 of the bisectors and the .25 of the area.
 template <typename T, int N0, int N1, int N2, int N3>
-LG_DEV void add_face_normal(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&nx)[8],
-                            T (&ny)[8], T (&nz)[8])
+LG_DEV void add_face_normal16(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&nx)[8],
+                              T (&ny)[8], T (&nz)[8])
 {
-   const T b0x = T(0.5) * (x[N3] + x[N2] - x[N1] - x[N0]);
-   const T b0y = T(0.5) * (y[N3] + y[N2] - y[N1] - y[N0]);
-   const T b0z = T(0.5) * (z[N3] + z[N2] - z[N1] - z[N0]);
-   const T b1x = T(0.5) * (x[N2] + x[N1] - x[N3] - x[N0]);
-   const T b1y = T(0.5) * (y[N2] + y[N1] - y[N3] - y[N0]);
-   const T b1z = T(0.5) * (z[N2] + z[N1] - z[N3] - z[N0]);
-   const T ax = T(0.25) * (b0y * b1z - b0z * b1y);
-   const T ay = T(0.25) * (b0z * b1x - b0x * b1z);
-   const T az = T(0.25) * (b0x * b1y - b0y * b1x);
+   const T b0x = x[N3] + x[N2] - x[N1] - x[N0];
+   const T b0y = y[N3] + y[N2] - y[N1] - y[N0];
+   const T b0z = z[N3] + z[N2] - z[N1] - z[N0];
+   const T b1x = x[N2] + x[N1] - x[N3] - x[N0];
+   const T b1y = y[N2] + y[N1] - y[N3] - y[N0];
+   const T b1z = z[N2] + z[N1] - z[N3] - z[N0];
+   const T ax = b0y * b1z - b0z * b1y;
+   const T ay = b0z * b1x - b0x * b1z;
+   const T az = b0x * b1y - b0y * b1x;
    nx[N0] += ax; nx[N1] += ax; nx[N2] += ax; nx[N3] += ax;
    ny[N0] += ay; ny[N1] += ay; ny[N2] += ay; ny[N3] += ay;
    nz[N0] += az; nz[N1] += az; nz[N2] += az; nz[N3] += az;
 }
 
-// CalcElemNodeNormals, lulesh.cc:419-474 (faces in the reference's order, so
-// each node's three contributions are summed in the same order).
+// 16 times CalcElemNodeNormals, lulesh.cc:419-474 (faces in the reference's
+// order, so each node's three contributions are summed in the same order).
 template <typename T>
-LG_DEV void node_normals(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&nx)[8],
-                         T (&ny)[8], T (&nz)[8])
+LG_DEV void node_normals16(const T (&x)[8], const T (&y)[8], const T (&z)[8], T (&nx)[8],
+                           T (&ny)[8], T (&nz)[8])
 {
 #pragma unroll
    for (int i = 0; i < 8; ++i) nx[i] = ny[i] = nz[i] = T(0);
-   add_face_normal<T, 0, 1, 2, 3>(x, y, z, nx, ny, nz);
-   add_face_normal<T, 0, 4, 5, 1>(x, y, z, nx, ny, nz);
-   add_face_normal<T, 1, 5, 6, 2>(x, y, z, nx, ny, nz);
-   add_face_normal<T, 2, 6, 7, 3>(x, y, z, nx, ny, nz);
-   add_face_normal<T, 3, 7, 4, 0>(x, y, z, nx, ny, nz);
-   add_face_normal<T, 4, 7, 6, 5>(x, y, z, nx, ny, nz);
+   add_face_normal16<T, 0, 1, 2, 3>(x, y, z, nx, ny, nz);
+   add_face_normal16<T, 0, 4, 5, 1>(x, y, z, nx, ny, nz);
+   add_face_normal16<T, 1, 5, 6, 2>(x, y, z, nx, ny, nz);
+   add_face_normal16<T, 2, 6, 7, 3>(x, y, z, nx, ny, nz);
+   add_face_normal16<T, 3, 7, 4, 0>(x, y, z, nx, ny, nz);
+   add_face_normal16<T, 4, 7, 6, 5>(x, y, z, nx, ny, nz);
 }
 
 // Bilinear six-node form of VoluDer (lulesh.cc:602-614) for one node whose
