@@ -87,12 +87,17 @@ struct PolicyFp64 {
    // 416 -> 411 us; FP64 582 -> 600 us, FMA 505 -> 534 us (the stress work
    // then waits for every load at once)
    static constexpr bool kPinElemFields = false;
+   // brick kernel: load the next element plane's coordinates and fields in
+   // the node phase of the current one (between its barriers).  FP64_FMA
+   // s=192: 498 -> 486 us; FP64 no gain (255 VGPRs)
+   static constexpr bool kPrefetchElem = false;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
 // translation unit); the types are those of PolicyFp64.
 struct PolicyFp64Fma : PolicyFp64 {
    static constexpr bool kHgVelEarly = true;
+   static constexpr bool kPrefetchElem = true;
 };
 
 struct PolicyMixed {
@@ -112,6 +117,7 @@ struct PolicyMixed {
    static constexpr bool kHgVelEarly = true;
    static constexpr bool kKinPrefetch = true;
    static constexpr bool kPinElemFields = true;
+   static constexpr bool kPrefetchElem = false; // 56 spills at 3 waves/SIMD
 };
 
 // Element i of a device array through a 32-bit byte offset: the access
