@@ -41,15 +41,19 @@ inline size_t align_up(size_t v) { return (v + kAlign - 1) & ~(kAlign - 1); }
 struct EnsTime {
    double *time, *deltatime, *dtcourant, *dthydro, *stoptime, *dtfixed, *dtmax, *multlb, *multub;
    int32_t *cycle, *active;
+   // the run's cycle limit, one value in device memory: set by each
+   // lgpu_ensemble_run, so that the cycle graphs do not depend on it
+   int32_t *maxCycle;
 };
 
 // TimeIncrement per member (lulesh.cc:166-222) and the loop test of
 // lulesh.cc:2745; a member whose step failed stops.
 __global__ void k_time_increment_ens(EnsTime t, const double *dtOut, const int32_t *err,
-                                     int nMembers, int maxCycle)
+                                     int nMembers)
 {
    const int m = blockIdx.x * blockDim.x + threadIdx.x;
    if (m >= nMembers || !t.active[m]) return;
+   const int maxCycle = *t.maxCycle;
    bool failed = false;
    for (int i = 0; i < kNumErrSites; ++i) failed = failed || (err[m * kNumErrSites + i] != 0);
    if (t.cycle[m] > 0) {
@@ -107,7 +111,6 @@ struct lgpu_ensemble {
    bool timing = false;
    hipGraph_t graph = nullptr;
    hipGraphExec_t exec = nullptr;
-   int graphMaxCycle = -1;
    // Brick path: every member is a reference brick; the fused nodal and
    // kinematics kernels run over all members' workgroups, node state
    // ping-pongs between two buffers (parity = which one is current), and a
@@ -156,10 +159,10 @@ void swap_nodes(lgpu_ensemble *e)
 }
 
 template <class P>
-void launch_cycle(lgpu_ensemble *e, int maxCycle)
+void launch_cycle(lgpu_ensemble *e)
 {
    hipLaunchKernelGGL(k_time_increment_ens, dim3((e->nMembers + 63) / 64), dim3(64), 0, e->stream,
-                      e->t, e->s.memberDtOut, e->s.memberErr, e->nMembers, maxCycle);
+                      e->t, e->s.memberDtOut, e->s.memberErr, e->nMembers);
    if (e->brick) {
       Launch<P>::nodal_brick_members(e->s, e->k, e->nodalBlocks, e->stream);
       swap_nodes(e);
@@ -173,25 +176,25 @@ void launch_cycle(lgpu_ensemble *e, int maxCycle)
    Launch<P>::dt_reduce(e->s, e->stream);
 }
 
-void enqueue_cycle(lgpu_ensemble *e, int maxCycle)
+void enqueue_cycle(lgpu_ensemble *e)
 {
    switch (e->precision) {
-   case LGPU_MIXED: launch_cycle<PolicyMixed>(e, maxCycle); break;
-   case LGPU_FP64_FMA: launch_cycle<PolicyFp64Fma>(e, maxCycle); break;
-   default: launch_cycle<PolicyFp64>(e, maxCycle); break;
+   case LGPU_MIXED: launch_cycle<PolicyMixed>(e); break;
+   case LGPU_FP64_FMA: launch_cycle<PolicyFp64Fma>(e); break;
+   default: launch_cycle<PolicyFp64>(e); break;
    }
 }
 
 // Per-kernel timing of one cycle (used by the autotuner and the timing
 // option): events around each launch.
 template <class P>
-int timed_cycle(lgpu_ensemble *e, int maxCycle, float ms[kENumKernels])
+int timed_cycle(lgpu_ensemble *e, float ms[kENumKernels])
 {
    hipEvent_t ev[kENumKernels + 1];
    for (auto &x : ev) ENS_HIP(e, hipEventCreate(&x));
    ENS_HIP(e, hipEventRecord(ev[0], e->stream));
    hipLaunchKernelGGL(k_time_increment_ens, dim3((e->nMembers + 63) / 64), dim3(64), 0, e->stream,
-                      e->t, e->s.memberDtOut, e->s.memberErr, e->nMembers, maxCycle);
+                      e->t, e->s.memberDtOut, e->s.memberErr, e->nMembers);
    ENS_HIP(e, hipEventRecord(ev[1], e->stream));
    if (e->brick) {
       Launch<P>::nodal_brick_members(e->s, e->k, e->nodalBlocks, e->stream);
@@ -224,12 +227,12 @@ int timed_cycle(lgpu_ensemble *e, int maxCycle, float ms[kENumKernels])
    return LGPU_OK;
 }
 
-int timed_cycle_any(lgpu_ensemble *e, int maxCycle, float ms[kENumKernels])
+int timed_cycle_any(lgpu_ensemble *e, float ms[kENumKernels])
 {
    switch (e->precision) {
-   case LGPU_MIXED: return timed_cycle<PolicyMixed>(e, maxCycle, ms);
-   case LGPU_FP64_FMA: return timed_cycle<PolicyFp64Fma>(e, maxCycle, ms);
-   default: return timed_cycle<PolicyFp64>(e, maxCycle, ms);
+   case LGPU_MIXED: return timed_cycle<PolicyMixed>(e, ms);
+   case LGPU_FP64_FMA: return timed_cycle<PolicyFp64Fma>(e, ms);
+   default: return timed_cycle<PolicyFp64>(e, ms);
    }
 }
 
@@ -241,13 +244,12 @@ void drop_graph(lgpu_ensemble *e)
    if (e->graph2) (void)hipGraphDestroy(e->graph2);
    e->exec = e->exec2 = nullptr;
    e->graph = e->graph2 = nullptr;
-   e->graphMaxCycle = -1;
 }
 
-int capture_cycle(lgpu_ensemble *e, int maxCycle, hipGraph_t &g, hipGraphExec_t &x)
+int capture_cycle(lgpu_ensemble *e, hipGraph_t &g, hipGraphExec_t &x)
 {
    ENS_HIP(e, hipStreamBeginCapture(e->stream, hipStreamCaptureModeThreadLocal));
-   enqueue_cycle(e, maxCycle);
+   enqueue_cycle(e);
    const hipError_t endErr = hipStreamEndCapture(e->stream, &g);
    ENS_HIP(e, endErr);
    ENS_HIP(e, hipGraphInstantiate(&x, g, nullptr, nullptr, 0));
@@ -257,21 +259,20 @@ int capture_cycle(lgpu_ensemble *e, int maxCycle, hipGraph_t &g, hipGraphExec_t 
 // One cycle graph; on the brick path a second one for the other node-buffer
 // parity (exec: captured at parity 0, exec2: at parity 1).  Capturing
 // advances the host's parity; it is put back afterwards.
-int build_graph(lgpu_ensemble *e, int maxCycle)
+int build_graph(lgpu_ensemble *e)
 {
    drop_graph(e);
    if (e->brick) {
       const int p0 = e->parity;
       if (p0) swap_nodes(e);
-      int r = capture_cycle(e, maxCycle, e->graph, e->exec);
-      if (!r) r = capture_cycle(e, maxCycle, e->graph2, e->exec2);
+      int r = capture_cycle(e, e->graph, e->exec);
+      if (!r) r = capture_cycle(e, e->graph2, e->exec2);
       if (e->parity != p0) swap_nodes(e);
       if (r) return r;
    } else {
-      const int r = capture_cycle(e, maxCycle, e->graph, e->exec);
+      const int r = capture_cycle(e, e->graph, e->exec);
       if (r) return r;
    }
-   e->graphMaxCycle = maxCycle;
    return LGPU_OK;
 }
 
@@ -506,6 +507,7 @@ int lgpu_ensemble_create(const lgpu_mesh_desc *members, int32_t nMembers, lgpu_e
    for (double **p : tf) add((void **)p, 8 * (size_t)nMembers);
    add((void **)&t.cycle, 4 * (size_t)nMembers);
    add((void **)&t.active, 4 * (size_t)nMembers);
+   add((void **)&t.maxCycle, 4);
    add((void **)&s.memberErr, 4 * (size_t)nMembers * kNumErrSites);
    add((void **)&s.memberDtOut, 16 * (size_t)nMembers);
    const size_t stateSlots = slots.size();
@@ -731,10 +733,11 @@ int lgpu_ensemble_run(lgpu_ensemble *e, lgpu_time_state *ts, int32_t maxCycle, i
       }
       ENS_HIP(e, hipMemcpyAsync(e->s.memberDtOut, dtout.data(), 16 * (size_t)M,
                                 hipMemcpyHostToDevice, e->stream));
+      ENS_HIP(e, hipMemcpyAsync(t.maxCycle, &maxCycle, 4, hipMemcpyHostToDevice, e->stream));
       ENS_HIP(e, hipStreamSynchronize(e->stream));
    }
-   if (e->useGraph && !e->timing && e->graphMaxCycle != maxCycle) {
-      const int r = build_graph(e, maxCycle);
+   if (e->useGraph && !e->timing && !e->exec) {
+      const int r = build_graph(e);
       if (r) return r;
    }
    const int batch = 8;
@@ -742,7 +745,7 @@ int lgpu_ensemble_run(lgpu_ensemble *e, lgpu_time_state *ts, int32_t maxCycle, i
       for (int b = 0; b < batch; ++b) {
          if (e->timing) {
             float ms[kENumKernels];
-            const int r = timed_cycle_any(e, maxCycle, ms);
+            const int r = timed_cycle_any(e, ms);
             if (r) return r;
             for (int i = 0; i < kENumKernels; ++i) {
                e->kLaunches[i] += 1;
@@ -752,7 +755,7 @@ int lgpu_ensemble_run(lgpu_ensemble *e, lgpu_time_state *ts, int32_t maxCycle, i
             const int r = launch_graph(e);
             if (r) return r;
          } else {
-            enqueue_cycle(e, maxCycle);
+            enqueue_cycle(e);
             ENS_HIP(e, hipGetLastError());
          }
       }
@@ -951,7 +954,10 @@ int lgpu_ensemble_autotune(lgpu_ensemble *e, const lgpu_time_state *ts, int32_t 
       e->kLaunches[i] = 0;
       e->kMs[i] = 0.0;
    }
+   // the cycle graphs of the chosen configuration, built here rather than
+   // in the first run
    drop_graph(e);
+   if (!r && e->useGraph && !e->timing) r = build_graph(e);
    (void)hipFree(snap);
    return r;
 }

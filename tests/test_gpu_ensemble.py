@@ -148,7 +148,9 @@ def test_autotune_keeps_state(generic):
     if generic:
         assert choice["block_force"] in (64, 128, 256)
     else:
-        assert choice["brick_chunk"] in (4, 8, 13, 22) and choice["kin_chunk"] in (4, 8, 12, 16)
+        # the autotuner's candidates (lgpu_ensemble_autotune)
+        assert choice["brick_chunk"] in (4, 6, 8, 11, 16, 22)
+        assert choice["kin_chunk"] in (4, 6, 8, 12, 16)
     st, _ = ens.run(10)
     assert st == 0
     ens.sync()
