@@ -397,6 +397,7 @@ int alloc_context(const CtxSpec &sp, lgpu_ctx **out)
                  kk.qstop, kk.monoq_max_slope, kk.monoq_limiter_mult, kk.qlc_monoq,
                  kk.qqc_monoq, kk.qqc, kk.eosvmax, kk.eosvmin, kk.pmin, kk.emin, kk.dvovmax,
                  kk.refdens};
+   c->k.fastEos = fast_eos_consts(c->k);
    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess)
       return fail(LGPU_EDEVICE);
 

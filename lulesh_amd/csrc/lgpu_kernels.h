@@ -27,6 +27,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <cmath>
 #include <type_traits>
 
 #include "lgpu_geom.h"
@@ -53,7 +54,19 @@ struct Consts {
    double e_cut, p_cut, q_cut, v_cut, u_cut;
    double hgcoef, ss4o3, qstop, monoq_max_slope, monoq_limiter_mult;
    double qlc_monoq, qqc_monoq, qqc, eosvmax, eosvmin, pmin, emin, dvovmax, refdens;
+   int32_t fastEos; // fast_eos_consts(*this), set by the host
 };
+
+// The constants k_eos's fast FP64 EOS repeats assume: e_cut and p_cut
+// positive (the values then clamped below by emin and pmin are never -0),
+// emin and pmin not -0 and of magnitude <= 2^100, refdens in [2^-64, 2^64].
+inline bool fast_eos_consts(const Consts &k)
+{
+   const double big = 0x1p100;
+   return k.e_cut > 0. && k.p_cut > 0. && std::fabs(k.emin) <= big && std::fabs(k.pmin) <= big &&
+          !(k.emin == 0. && std::signbit(k.emin)) && !(k.pmin == 0. && std::signbit(k.pmin)) &&
+          k.refdens >= 0x1p-64 && k.refdens <= 0x1p64;
+}
 
 struct PolicyFp64 {
    using force_t = double;  // force integration arithmetic

@@ -93,6 +93,39 @@ def test_lagrange_elements_and_dt(s, r, b, c, warm, generic):
     assert dth == o.get("dthydro")
 
 
+@pytest.mark.parametrize("generic", PATHS)
+def test_eos_waves_outside_the_fast_form_match(generic):
+    """k_eos repeats the FP64 EOS of a wave in a fast form (short correctly
+    rounded reciprocals, lower clamps as maximum instructions) when the
+    wave's inputs bound every intermediate value, and otherwise in the
+    general form.  Energies and pressures of magnitude 2^101 (beyond the
+    bound, so the wave is redone in the general form) and 2^99 (within it)
+    must both give the oracle's bits."""
+    o = OracleDomain(8, 21, 2, 2)
+    advance(o, 6)
+    o.call("time_increment")
+    assert o.call("lagrange_nodal") == 0
+    e, p = o.real("e"), o.real("p")
+    e[::37] = 2.0 ** 101
+    e[5::41] = 2.0 ** 99
+    p[11::53] = -2.0 ** 101
+    p[17::59] = 2.0 ** 99
+    ctx = context_from_oracle(o, generic=generic)
+    ctx.set_option(1, 1)
+    upload_state(ctx, o)
+    dt = o.get("deltatime")
+    assert ctx.lagrange_elements(dt) == 0
+    st, dtc, dth = ctx.time_constraints()
+    assert st == 0
+    assert o.call("lagrange_elements") == 0
+    o.call("calc_time_constraints")
+    for f in ELEM_OUT:
+        got, want = ctx.download(f), o.real(f)
+        assert np.array_equal(got, want), (f, max_rel(got, want))
+    assert dtc == o.get("dtcourant")
+    assert dth == o.get("dthydro")
+
+
 @pytest.mark.parametrize("s,chunk,kin_chunk,rows", [(7, 8, 8, 16), (16, 3, 1, 16), (33, 5, 7, 16),
                                                     (31, 16, 31, 16), (17, 6, 4, 8), (30, 13, 8, 8)])
 def test_brick_and_generic_paths_bit_identical(s, chunk, kin_chunk, rows):
