@@ -76,12 +76,13 @@ def kernel_bytes(name: str, s: int, precision: str, brick: bool = True) -> float
         # fused brick nodal pass: x..zd read 48r, p q ss, elemMass v volo 24,
         # nodalMass 8r, node BC mask 1r, x..zd written 48r
         "k_nodal_brick": 48 * r + 3 * n4 + 24 + 8 * r + 1 * r + 48 * r,
-        # nodelist, x..zd 48r, volo v 16 -> vnew delv arealg vdov 32, 6 gradients
-        "k_kin": nl + 48 * r + 16 + 32 + 6 * n4,
-        "k_kin_brick": 48 * r + 16 + 32 + 6 * n4,
+        # nodelist, x..zd 48r, volo 8 -> vnew arealg vdov 24, 6 gradients
+        # (delv = vnew - v is formed by k_eos, not stored)
+        "k_kin": nl + 48 * r + 8 + 24 + 6 * n4,
+        "k_kin_brick": 48 * r + 8 + 24 + 6 * n4,
         # perm 4, faces, 6 gradients, p q, vdov elemMass arealg volo vnew v e
-        # delv 64 -> p q ss, e v 16
-        "k_eos": 4 + faces + 6 * n4 + 2 * n4 + 64 + 3 * n4 + 16,
+        # 56 -> p q ss, e v 16
+        "k_eos": 4 + faces + 6 * n4 + 2 * n4 + 56 + 3 * n4 + 16,
     }
     return per_elem.get(name, 0.0) * E
 

@@ -140,7 +140,7 @@ void lgpu_destroy(lgpu_ctx *ctx);
 int lgpu_upload(lgpu_ctx *ctx, int field, const double *host);
 
 /* Synchronous device -> host copy of one real field.  Fields the step does
- * not keep (fx.., xdd.., ql, qq) hold their last value only while
+ * not keep (fx.., xdd.., ql, qq, delv) hold their last value only while
  * lgpu_set_option(LGPU_OPT_KEEP_DIAGNOSTICS) is on. */
 int lgpu_download(lgpu_ctx *ctx, int field, double *host);
 
@@ -219,7 +219,7 @@ int lgpu_create_sedov(const lgpu_sedov_desc *desc, lgpu_ctx **out, lgpu_time_sta
 
 /* Options. */
 enum lgpu_option {
-   LGPU_OPT_KEEP_DIAGNOSTICS = 1, /* also store fx,fy,fz,xdd,ydd,zdd,ql,qq  */
+   LGPU_OPT_KEEP_DIAGNOSTICS = 1, /* also store fx..,xdd..,ql,qq,delv      */
    LGPU_OPT_KERNEL_TIMING = 2,    /* time every kernel with HIP events       */
    LGPU_OPT_BLOCK_SIZE = 3,       /* threads per block of the element/node
                                      kernels of the generic path: 64, 128
@@ -283,7 +283,9 @@ const char *lgpu_last_error(lgpu_ctx *ctx);
  * run of that member.  When every member is the reference's brick (and
  * members[0].flags lacks LGPU_FLAG_GENERIC) the fused brick kernels run over
  * all members' workgroups in one launch each; otherwise the CSR-gather
- * kernels are used.
+ * kernels are used.  Ensembles keep no diagnostics: the fields a single
+ * context keeps only with LGPU_OPT_KEEP_DIAGNOSTICS (fx.., xdd.., ql, qq,
+ * delv) are not written.
  * ---------------------------------------------------------------------- */
 typedef struct lgpu_ensemble lgpu_ensemble;
 

@@ -304,9 +304,10 @@ const int kStateFields[] = {LGPU_X, LGPU_Y, LGPU_Z, LGPU_XD, LGPU_YD, LGPU_ZD, L
                             LGPU_E, LGPU_P, LGPU_Q, LGPU_V, LGPU_VOLO, LGPU_SS, LGPU_ELEMMASS};
 // What a cycle writes: the fields a sync brings back.  nodalMass, volo and
 // elemMass stay the host's FP64 inputs (the mixed form keeps elemMass in FP32
-// on the device; its download would be the rounded copy).
+// on the device; its download would be the rounded copy).  delv, like ql and
+// qq, is formed and used inside one cycle and not kept.
 const int kStepFields[] = {LGPU_X, LGPU_Y, LGPU_Z, LGPU_XD, LGPU_YD, LGPU_ZD, LGPU_E, LGPU_P,
-                           LGPU_Q, LGPU_V, LGPU_SS, LGPU_VNEW, LGPU_DELV, LGPU_VDOV, LGPU_AREALG};
+                           LGPU_Q, LGPU_V, LGPU_SS, LGPU_VNEW, LGPU_VDOV, LGPU_AREALG};
 } // namespace
 
 int Domain::AttachDevice(int precision, int flags)

@@ -303,6 +303,7 @@ int enable_diagnostics(lgpu_ctx *c)
    for (int i = 0; i < 6; ++i) *node[i] = reinterpret_cast<double *>(p + i * nn);
    c->s.ql = reinterpret_cast<double *>(p + 6 * nn);
    c->s.qq = reinterpret_cast<double *>(p + 6 * nn + ne);
+   c->s.delvOut = c->s.delv;
    return LGPU_OK;
 }
 
@@ -1000,7 +1001,7 @@ int lgpu_set_option(lgpu_ctx *c, int option, int64_t value)
          if (r) return r;
       } else {
          c->s.fx = c->s.fy = c->s.fz = c->s.xdd = c->s.ydd = c->s.zdd = nullptr;
-         c->s.ql = c->s.qq = nullptr;
+         c->s.ql = c->s.qq = c->s.delvOut = nullptr;
          HIP_OR_RETURN(c, hipStreamSynchronize(c->stream));
          if (c->diag) (void)hipFree(c->diag);
          c->diag = nullptr;

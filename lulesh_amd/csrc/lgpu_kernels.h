@@ -92,8 +92,9 @@ struct PolicyFp64 {
    // loaded one component ahead.  FP64 s=192: after, 598 us against 748 (256
    // VGPRs before); FMA and mixed: before, 1.4% and 0.5% faster
    static constexpr bool kHgVelEarly = false;
-   // k_kin_brick: load volo and v a plane ahead.  Mixed s=192: 190 -> 181 us;
-   // FP64: 249 -> 269 us (the extra registers cost it its third wave)
+   // k_kin_brick: load the element's volo a plane ahead (measured when v was
+   // loaded with it: mixed s=192 190 -> 181 us; FP64 249 -> 269 us, the
+   // extra registers cost it its third wave)
    static constexpr bool kKinPrefetch = false;
    // brick kernel: a scheduling barrier after the element-field loads (p, q,
    // v, volo, ss, elemMass), so that none is sunk to its use.  Mixed s=192:
@@ -198,6 +199,7 @@ struct DevState {
    double *vnew, *delv;
    double *arealg, *vdov;
    double *ql, *qq; // diagnostics (may be null)
+   double *delvOut; // diagnostics: delv = vnew - v, stored by k_eos (may be null)
    void *delx_xi, *delx_eta, *delx_zeta, *delv_xi, *delv_eta, *delv_zeta; // grad_t
    // corner planes: [(component*2 + kind)*8 + corner][elem], kind 0 stress, 1 hourglass
    void *corner;
