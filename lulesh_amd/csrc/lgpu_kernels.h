@@ -92,10 +92,6 @@ struct PolicyFp64 {
    // loaded one component ahead.  FP64 s=192: after, 598 us against 748 (256
    // VGPRs before); FMA and mixed: before, 1.4% and 0.5% faster
    static constexpr bool kHgVelEarly = false;
-   // k_kin_brick: load the element's volo a plane ahead (measured when v was
-   // loaded with it: mixed s=192 190 -> 181 us; FP64 249 -> 269 us, the
-   // extra registers cost it its third wave)
-   static constexpr bool kKinPrefetch = false;
    // brick kernel: a scheduling barrier after the element-field loads (p, q,
    // v, volo, ss, elemMass), so that none is sunk to its use.  Mixed s=192:
    // 416 -> 411 us; FP64 582 -> 600 us, FMA 505 -> 534 us (the stress work
@@ -129,7 +125,6 @@ struct PolicyMixed {
    static constexpr bool kPrefetchNode = true;
    static constexpr bool kOffset32 = true;
    static constexpr bool kHgVelEarly = true;
-   static constexpr bool kKinPrefetch = true;
    static constexpr bool kPinElemFields = true;
    static constexpr bool kPrefetchElem = false; // 56 spills at 3 waves/SIMD
 };
