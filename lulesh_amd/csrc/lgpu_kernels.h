@@ -101,6 +101,9 @@ struct PolicyFp64 {
    // the node phase of the current one (between its barriers).  FP64_FMA
    // s=192: 498 -> 486 us; FP64 no gain (255 VGPRs)
    static constexpr bool kPrefetchElem = false;
+   // k_eos waves per SIMD: FP64 at 7 (at most 72 VGPRs); 8 (64 VGPRs, four
+   // spilled) ran 326 -> 332 us at s=192
+   static constexpr int kEosWaves = 7;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
@@ -127,6 +130,8 @@ struct PolicyMixed {
    static constexpr bool kHgVelEarly = true;
    static constexpr bool kPinElemFields = true;
    static constexpr bool kPrefetchElem = false; // 56 spills at 3 waves/SIMD
+   // k_eos at 8 waves per SIMD (at most 64 VGPRs): 214 -> 211 us at s=192
+   static constexpr int kEosWaves = 8;
 };
 
 // Element i of a device array through a 32-bit byte offset: the access
