@@ -24,14 +24,16 @@ def summary(s, r, b, c, its, m=1.0) -> dict:
     return summaries()[f"s{s}_r{r}_b{b}_c{c}_i{its}_m{m:.17g}"]
 
 
-def context_from_oracle(o: OracleDomain, precision=la.FP64, generic=False) -> la.DeviceContext:
+def context_from_oracle(o: OracleDomain, precision=la.FP64, generic=False,
+                        constants=None) -> la.DeviceContext:
     return la.DeviceContext(
         nodelist=o.ints("nodelist"), lxim=o.ints("lxim"), lxip=o.ints("lxip"),
         letam=o.ints("letam"), letap=o.ints("letap"), lzetam=o.ints("lzetam"),
         lzetap=o.ints("lzetap"), elemBC=o.ints("elemBC"), nodeElemStart=o.ints("nodeElemStart"),
         nodeElemCornerList=o.ints("nodeElemCornerList"), symmX=o.ints("symmX"),
         symmY=o.ints("symmY"), symmZ=o.ints("symmZ"), regions=o.region_lists(), cost=o.cost,
-        size=o.s, precision=precision, flags=la.native.FLAG_GENERIC if generic else 0)
+        size=o.s, precision=precision, flags=la.native.FLAG_GENERIC if generic else 0,
+        constants=constants)
 
 
 def upload_state(ctx: la.DeviceContext, o: OracleDomain, fields=STATE_FIELDS) -> None:

@@ -126,6 +126,34 @@ def test_eos_waves_outside_the_fast_form_match(generic):
     assert dth == o.get("dthydro")
 
 
+def test_eos_fast_form_equals_general_form():
+    """The fast FP64 EOS repeats run only with constants that pass
+    fast_eos_consts; pmin = -0.0 fails it (its clamp gives the same values up
+    to the sign of zero) and so runs every wave in the general form.  From one
+    mid-run state, the elements phase with the default constants and with
+    pmin = -0.0 gives equal fields (np.array_equal, -0 == 0) and equal dt."""
+    o = OracleDomain(10, 21, 2, 2)
+    advance(o, 30)
+    o.call("time_increment")
+    assert o.call("lagrange_nodal") == 0
+    dt = o.get("deltatime")
+    runs = []
+    for pmin in (0.0, -0.0):
+        k = la.native.default_constants()
+        k.pmin = pmin
+        ctx = context_from_oracle(o, constants=k)
+        ctx.set_option(1, 1)
+        upload_state(ctx, o)
+        assert ctx.lagrange_elements(dt) == 0
+        st, dtc, dth = ctx.time_constraints()
+        assert st == 0
+        runs.append(({f: ctx.download(f) for f in ELEM_OUT}, dtc, dth))
+    (fast, dtc0, dth0), (general, dtc1, dth1) = runs
+    for f in ELEM_OUT:
+        assert np.array_equal(fast[f], general[f]), (f, max_rel(fast[f], general[f]))
+    assert dtc0 == dtc1 and dth0 == dth1
+
+
 @pytest.mark.parametrize("s,chunk,kin_chunk,rows", [(7, 8, 8, 16), (16, 3, 1, 16), (33, 5, 7, 16),
                                                     (31, 16, 31, 16), (17, 6, 4, 8), (30, 13, 8, 8)])
 def test_brick_and_generic_paths_bit_identical(s, chunk, kin_chunk, rows):
