@@ -6,11 +6,11 @@
 //                        per-corner forces to corner planes
 //   k_node   (nodes)     deterministic corner gather in nodeElemCornerList
 //                        order, acceleration, symmetry BCs, velocity, position
-//   k_kin    (elements)  volume, vnew/delv, characteristic length, strain
-//                        rate, monotonic-Q gradients
+//   k_kin    (elements)  volume, vnew, characteristic length, strain rate,
+//                        monotonic-Q gradients
 //   k_eos    (region-ordered elements)  qstop check, monotonic-Q limiter,
-//                        EOS with its region repeat count, sound speed,
-//                        volume update, Courant/hydro block minima
+//                        delv = vnew - v, EOS with its region repeat count,
+//                        sound speed, volume update, Courant/hydro block minima
 //   k_dt_slots / k_dt_reduce (one block per problem) Courant/hydro minima
 //                        over the k_eos slots (one problem: atomic minima
 //                        into 256 slot pairs) or block partials (ensembles)
