@@ -104,6 +104,10 @@ struct PolicyFp64 {
    // k_eos waves per SIMD: FP64 at 7 (at most 72 VGPRs); 8 (64 VGPRs, four
    // spilled) ran 326 -> 332 us at s=192
    static constexpr int kEosWaves = 7;
+   // brick kernel: the node phase at raised wave priority (s_setprio 2).
+   // s=192: FP64 576.4 -> 568.9 us, FP64_FMA 489 -> 476.5 us; mixed
+   // 417.5 -> 420.4 us
+   static constexpr bool kNodePhasePrio = true;
 };
 
 // The FP64 policy compiled with multiply-add contraction (its own
@@ -132,6 +136,7 @@ struct PolicyMixed {
    static constexpr bool kPrefetchElem = false; // 56 spills at 3 waves/SIMD
    // k_eos at 8 waves per SIMD (at most 64 VGPRs): 214 -> 211 us at s=192
    static constexpr int kEosWaves = 8;
+   static constexpr bool kNodePhasePrio = false;
 };
 
 // Element i of a device array through a 32-bit byte offset: the access
