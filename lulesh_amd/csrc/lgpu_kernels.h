@@ -105,7 +105,8 @@ struct PolicyFp64 {
    // spilled) ran 326 -> 332 us at s=192
    static constexpr int kEosWaves = 7;
    // brick kernel: the node phase at raised wave priority (s_setprio 2).
-   // s=192: FP64 576.4 -> 568.9 us, FP64_FMA 489 -> 476.5 us; mixed
+   // s=192, every kernel timed: FP64 576.4 -> 568.9 us, FP64_FMA 489 ->
+   // 476.5 us (device loop: FP64 unchanged, FP64_FMA 484 -> 472 us); mixed
    // 417.5 -> 420.4 us
    static constexpr bool kNodePhasePrio = true;
 };
